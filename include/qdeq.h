@@ -1,0 +1,174 @@
+/*
+ * qdeq.h -- C ABI of the MI355X-native quasi-dynamic earthquake-cycle engine
+ * (libqdeq.so, built from qdeqcycle_fdm_amd/csrc for gfx950).
+ *
+ * The entry points replace the reference's hot-path call surface
+ * (XuSun-SUSTech/QDEqCycle_FDM functions.h:11-45): adaptive RK43 stepping that
+ * calls the antiplane elliptic multigrid solve and then the fault traction /
+ * rate-and-state update.  Each replaced function is cited next to its entry
+ * point.
+ *
+ * Conventions (the reference's, so callers can hand over their arrays):
+ *   - FP64 everywhere at the boundary ("Real = double", struct.h:4);
+ *   - grid arrays are nx*nz, x fastest, one ghost layer: idx = i + k*nx with
+ *     nx = ni+2, nz = nk+2 (odeSolver.cpp:24-37); column 1 is the fault,
+ *     column ni the remote edge; row 1 the bottom, row nk the free surface;
+ *   - mu_equi is the doubled (2nx-1)*(2nz-1) equivalent-medium array
+ *     (odeSolver.cpp:47,90);
+ *   - fault arrays have length nz, valid on rows 1..nk (Alloc.cpp:3-40);
+ *   - units: m, GPa, MPa, MPa*s/m, m/s, s (fault.cpp:153, coord.cpp:18).
+ * Differences from the reference:
+ *   - every entry point returns a qdeq_status instead of calling exit()
+ *     (helper_cuda.h:540-555, fault.cpp:218);
+ *   - array arguments carry a pointer-space flag (QDEQ_HOST / QDEQ_DEVICE);
+ *   - all solver state lives in a context (the reference keeps it in
+ *     file-scope globals, odeSolver.cpp:6-12), so several contexts may coexist;
+ *     one context must not be used from two host threads at once.
+ */
+#ifndef QDEQ_H
+#define QDEQ_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum qdeq_status {
+  QDEQ_OK = 0,
+  QDEQ_ERR_ARG = 1,               /* invalid argument or size */
+  QDEQ_ERR_HIP = 2,               /* HIP runtime error */
+  QDEQ_ERR_NAN_FRICTION = 3,      /* NR produced NaN twice (fault.cpp:211-219) */
+  QDEQ_ERR_MG_NOT_CONVERGED = 4,  /* V-cycle cap reached before tolerance */
+  QDEQ_ERR_STATE = 5,             /* call out of order (e.g. step before setup) */
+  QDEQ_ERR_NOMEM = 6
+} qdeq_status;
+
+enum { QDEQ_HOST = 0, QDEQ_DEVICE = 1 };
+/* elliptic stop rule: reference ||r||_2 <= tol (odeSolver.cpp:689-696), or
+ * ||r||_2 <= tol * ||r(u_interior = 0)||_2 (boundary-data residual) */
+enum { QDEQ_TOL_ABS = 0, QDEQ_TOL_REL = 1 };
+/* V-cycle arithmetic: plain FP64, or FP32 V-cycles inside FP64 iterative
+ * refinement (FP64 residual and update) */
+enum { QDEQ_PREC_FP64 = 0, QDEQ_PREC_FP32_IR = 1 };
+/* V-cycle shape: the reference's exactly (odeSolver.cpp:1101-1147, incl. the
+ * uncorrected top row), or the same cycle with the sweep counts of the
+ * config and a bilinear correction of the top row as well */
+enum { QDEQ_CYCLE_REFERENCE = 0, QDEQ_CYCLE_FAST = 1 };
+
+typedef struct qdeq_config {
+  int ni, nk;        /* interior nodes in x and z (SetParams.cpp:21-22) */
+  double dh;         /* grid spacing in m (SetParams.cpp:23) */
+  int n_levels;      /* multigrid levels (odeSolver.cpp:3) */
+  int n_pre, n_post, n_coarse; /* sweeps (odeSolver.cpp:4: 32 each) */
+  double relax;      /* RBGS relaxation (odeSolver.cpp:2) */
+  int cycle;         /* QDEQ_CYCLE_* */
+  int precision;     /* QDEQ_PREC_* */
+  int tol_mode;      /* QDEQ_TOL_* */
+  double tol;        /* 1e-9 absolute in the reference (odeSolver.cpp:689) */
+  int max_vcycles;   /* cap per solve (> 0); the reference has none */
+  double ini_dt, mindt, maxdt, totTol, tmax;   /* SetParams.cpp:5-18 */
+  double f0, V0, L, Vini;                      /* fault.cpp:85-89 */
+  int device;        /* HIP device ordinal */
+} qdeq_config;
+
+typedef struct qdeq_ctx qdeq_ctx;
+
+/* Per accepted step record (the reference prints it, odeSolver.cpp:1636). */
+typedef struct qdeq_step_info {
+  double t, dt;              /* time after the step, next step size */
+  double totErr;             /* embedded error of the accepted step */
+  double errA0, errA1;       /* controller history (odeSolver.cpp:1352) */
+  double V_max, Ts_max;      /* maxval (fault.cpp:22-52) */
+  int it;                    /* accepted steps so far */
+  int attempts;              /* attempts of the last step */
+  long long vcycles;         /* V-cycles (FP64 ones, or FP32 ones in IR) */
+  long long solves;          /* elliptic solves */
+  long long ir_sweeps;       /* refinement iterations (FP32_IR only) */
+} qdeq_step_info;
+
+/* Fills the reference's settings (SetParams.cpp, odeSolver.cpp:2-4, 689). */
+void qdeq_default_config(qdeq_config *cfg);
+
+/* Creates a context on cfg->device: level hierarchy (odeSolver.cpp:1384-1420),
+ * operator coefficients and device buffers.  mu: nodal shear modulus, nx*nz
+ * host array (MEDIUM.mu).  mu_equi: doubled equivalent-medium host array, or
+ * NULL to build it on the device from mu (SetMedium, odeSolver.cpp:337-589). */
+int qdeq_create(const qdeq_config *cfg, const double *mu,
+                const double *mu_equi, qdeq_ctx **out);
+int qdeq_destroy(qdeq_ctx *ctx);
+/* Message of the last failing call on ctx ("" if none). */
+const char *qdeq_last_error(const qdeq_ctx *ctx);
+/* Message of the last failing qdeq_create / context-free call. */
+const char *qdeq_global_error(void);
+
+/* Writes the doubled equivalent-medium array the context uses (built on the
+ * device when qdeq_create got NULL).  where: QDEQ_HOST / QDEQ_DEVICE. */
+int qdeq_get_mu_equi(qdeq_ctx *ctx, double *mu_equi, int where);
+/* Context-free device build of SetMedium's mu_equi (odeSolver.cpp:337-589)
+ * from the nodal nx*nz host array mu, on HIP device `device`. */
+int qdeq_build_mu_equi(int ni, int nk, const double *mu, double *mu_equi,
+                       int device);
+
+/* ---- drop-in replacements of the reference's hot-path functions ---- */
+
+/* ode_solve (odeSolver.cpp:605-609) = multigrid (652-720) + extend_crew
+ * (612-650): solves div(mu grad u) = 0 for the interior of U with U's fault
+ * and remote columns as Dirichlet data, starting from U (warm start), then
+ * extrapolates the ghost layer.  U: nx*nz.  Outputs (optional, may be NULL):
+ * V-cycles done, boundary-data (REL) or initial (ABS) residual norm, final
+ * residual norm. */
+int qdeq_ode_solve(qdeq_ctx *ctx, double *U, int where, int *vcycles,
+                   double *res0, double *res);
+
+/* setBoundary (odeSolver.cpp:1150-1188): fault column = slip/2, remote column
+ * = Vini*(t + ratio*dt)/2 + slip_shift, mirrored ghost rows.  U: nx*nz;
+ * slip, slip_shift: nz. */
+int qdeq_set_boundary(qdeq_ctx *ctx, double *U, const double *slip,
+                      const double *slip_shift, double t, double ratio,
+                      double dt, int where);
+
+/* eval_stress_on_fault + partial1st (fault.cpp:139-164): Ts = mu*du/dx*1e3
+ * at the fault column (one-sided 4-point stencil), Tn = tau_n0. */
+int qdeq_eval_stress(qdeq_ctx *ctx, const double *U, const double *mu,
+                     const double *tau_n0, double *Ts, double *Tn, int where);
+
+/* NRsearch (fault.cpp:169-232): per node, Newton on the asinh variable for
+ * the slip rate with radiation damping.  V holds the previous slip rate on
+ * entry (the Newton start) and the new one on exit.  Returns
+ * QDEQ_ERR_NAN_FRICTION where the reference would exit(0). */
+int qdeq_nr_search(qdeq_ctx *ctx, const double *Ts, const double *Tn,
+                   const double *psi, const double *a, const double *eta,
+                   double *V, int where);
+
+/* cal_dpsi (odeSolver.cpp:1190-1206): aging-law state rate. */
+int qdeq_cal_dpsi(qdeq_ctx *ctx, const double *psi, const double *V,
+                  const double *b, double *dpsi, int where);
+
+/* computeError (odeSolver.cpp:1211-1238) on host arrays of length nk+2. */
+double qdeq_compute_error(int nk, const double *slip3, const double *psi3,
+                          const double *slip4, const double *psi4);
+/* computeStepSize (odeSolver.cpp:1241-1271). errA: 2 doubles. */
+double qdeq_compute_step_size(int it, double dt, double mindt, double maxdt,
+                              double totErr, double totTol,
+                              const double *errA);
+
+/* ---- RK43 driver (odeSolver.cpp:1277-1674), device resident ---- */
+
+/* fault_init (fault.cpp:116-133) + the t=0 stage (odeSolver.cpp:1428-1437).
+ * a, b, eta, tau_n0: nz host arrays; U0: nx*nz host initial displacement
+ * (ghosts as extend_crew leaves them).  Slip = 2 U0(fault), SlipShift =
+ * U0(remote), psi = f0 + b ln(V0/Vini). */
+int qdeq_rk43_setup(qdeq_ctx *ctx, const double *a, const double *b,
+                    const double *eta, const double *tau_n0, const double *U0);
+/* Advances nsteps accepted steps (odeSolver.cpp:1462-1633).  If info is not
+ * NULL it receives the record of the last step. */
+int qdeq_rk43_step(qdeq_ctx *ctx, int nsteps, qdeq_step_info *info);
+/* Copies the current state to host arrays (any may be NULL): U (nx*nz, with
+ * ghosts as ode_solve leaves them), slip, psi, V, dpsi, Ts (nz). */
+int qdeq_rk43_get_state(qdeq_ctx *ctx, qdeq_step_info *info, double *U,
+                        double *slip, double *psi, double *V, double *dpsi,
+                        double *Ts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QDEQ_H */

@@ -1,0 +1,603 @@
+// api.hip -- context, drop-in entry points and the device-resident RK43
+// driver behind include/qdeq.h.
+//
+// Host control flow follows XuSun-SUSTech/QDEqCycle_FDM RK43
+// (odeSolver.cpp:1277-1674); the per-node and per-grid-point arithmetic runs
+// in the kernels of fault.hip and mg.hip.  The reference's exit() paths
+// become status codes.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "qdeq_internal.h"
+
+namespace {
+thread_local std::string g_error;
+}
+
+namespace qdeq {
+
+int set_error(qdeq_ctx *c, int code, const std::string &msg) {
+  if (c)
+    c->err = msg;
+  else
+    g_error = msg;
+  return code;
+}
+
+}  // namespace qdeq
+
+using namespace qdeq;
+
+template <typename T>
+static hipError_t alloc_zero(T **p, size_t n, hipStream_t s) {
+  hipError_t e = hipMalloc(p, n * sizeof(T));
+  if (e != hipSuccess) return e;
+  return hipMemsetAsync(*p, 0, n * sizeof(T), s);
+}
+
+extern "C" {
+
+void qdeq_default_config(qdeq_config *c) {
+  std::memset(c, 0, sizeof(*c));
+  c->ni = 801;  // SetParams.cpp:21-23
+  c->nk = 801;
+  c->dh = 30.0;
+  c->n_levels = 6;  // odeSolver.cpp:3
+  c->n_pre = c->n_post = c->n_coarse = 32;  // odeSolver.cpp:4
+  c->relax = 1.0;   // odeSolver.cpp:2
+  c->cycle = QDEQ_CYCLE_REFERENCE;
+  c->precision = QDEQ_PREC_FP64;
+  c->tol_mode = QDEQ_TOL_ABS;
+  c->tol = 1.0e-9;  // odeSolver.cpp:689
+  c->max_vcycles = 1000;
+  c->ini_dt = 1e-3;  // SetParams.cpp:14-17
+  c->mindt = 1e-5;
+  c->maxdt = 1e8;
+  c->totTol = 1e-7;
+  c->tmax = 0 + 60.0 * (1 + 60.0 * (0 + 24.0 * (0 + 30.0 * (0 + 12.0 * 3000))));
+  c->f0 = 0.6;  // fault.cpp:85-89
+  c->V0 = 1e-6;
+  c->L = 0.008;
+  c->Vini = 1e-9;
+  c->device = 0;
+}
+
+const char *qdeq_last_error(const qdeq_ctx *ctx) {
+  return ctx ? ctx->err.c_str() : g_error.c_str();
+}
+
+const char *qdeq_global_error(void) { return g_error.c_str(); }
+
+static void free_ctx(qdeq_ctx *c) {
+  if (!c) return;
+  mg_destroy_graphs(c);
+  for (int l = 0; l < QDEQ_MAX_LEVELS; l++) {
+    (void)hipFree(c->d[l].u); (void)hipFree(c->d[l].f); (void)hipFree(c->d[l].r);
+    (void)hipFree(c->d[l].cx); (void)hipFree(c->d[l].cz);
+    (void)hipFree(c->s[l].u); (void)hipFree(c->s[l].f); (void)hipFree(c->s[l].r);
+    (void)hipFree(c->s[l].cx); (void)hipFree(c->s[l].cz);
+  }
+  (void)hipFree(c->mu_equi);
+  (void)hipFree(c->partial);
+  (void)hipFree(c->scal);
+  (void)hipFree(c->flags);
+  (void)hipFree(c->stage_U);
+  for (auto &p : c->stage_v) (void)hipFree(p);
+  (void)hipFree(c->fb.base);
+  if (c->scal_host) (void)hipHostFree(c->scal_host);
+  if (c->flags_host) (void)hipHostFree(c->flags_host);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+#define CREATE_HIP(call)                                                   \
+  do {                                                                     \
+    hipError_t e_ = (call);                                                \
+    if (e_ != hipSuccess) {                                                \
+      set_error(nullptr, QDEQ_ERR_HIP,                                     \
+                std::string(#call) + ": " + hipGetErrorString(e_));        \
+      free_ctx(c);                                                         \
+      return QDEQ_ERR_HIP;                                                 \
+    }                                                                      \
+  } while (0)
+
+int qdeq_create(const qdeq_config *cfg, const double *mu,
+                const double *mu_equi, qdeq_ctx **out) {
+  if (!cfg || !mu || !out)
+    return set_error(nullptr, QDEQ_ERR_ARG, "qdeq_create: null argument");
+  *out = nullptr;
+  if (cfg->ni < 5 || cfg->nk < 5 || cfg->dh <= 0 || cfg->n_levels < 1 ||
+      cfg->n_levels > QDEQ_MAX_LEVELS || cfg->n_pre < 0 || cfg->n_post < 0 ||
+      cfg->n_coarse < 1 || cfg->max_vcycles < 1 || cfg->tol <= 0)
+    return set_error(nullptr, QDEQ_ERR_ARG, "qdeq_create: invalid config");
+  qdeq_ctx *c = new qdeq_ctx();
+  c->cfg = *cfg;
+  c->nx = cfg->ni + 2;
+  c->nz = cfg->nk + 2;
+  c->L = cfg->n_levels;
+  {
+    int n = cfg->nk, k = cfg->ni, fc = 1;
+    double h = cfg->dh;
+    for (int l = 0; l < c->L; l++) {
+      if (n < 3 || k < 3) {
+        delete c;
+        return set_error(nullptr, QDEQ_ERR_ARG,
+                         "qdeq_create: too many levels for the grid");
+      }
+      c->lv[l] = LevelDims{n, k, (k + 63) / 64 * 64, fc, h};
+      n = (n - 1) / 2 + 1;
+      k = (k - 1) / 2 + 1;
+      h *= 2;
+      fc *= 2;
+    }
+  }
+  const double dh = cfg->dh;  // partial1st (fault.cpp:162)
+  c->c_stress[0] = -24.0 / (17.0 * dh);
+  c->c_stress[1] = 59.0 / (34.0 * dh);
+  c->c_stress[2] = -4.0 / (17.0 * dh);
+  c->c_stress[3] = -3.0 / (34.0 * dh);
+
+  CREATE_HIP(hipSetDevice(cfg->device));
+  CREATE_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  for (int l = 0; l < c->L; l++) {
+    const LevelDims &d = c->lv[l];
+    const size_t n = (size_t)d.N * d.P, nz = (size_t)(d.N + 1) * d.P;
+    CREATE_HIP(alloc_zero(&c->d[l].u, n, c->stream));
+    CREATE_HIP(alloc_zero(&c->d[l].f, n, c->stream));
+    CREATE_HIP(alloc_zero(&c->d[l].r, n, c->stream));
+    CREATE_HIP(alloc_zero(&c->d[l].cx, n, c->stream));
+    CREATE_HIP(alloc_zero(&c->d[l].cz, nz, c->stream));
+    CREATE_HIP(alloc_zero(&c->s[l].u, n, c->stream));
+    CREATE_HIP(alloc_zero(&c->s[l].f, n, c->stream));
+    CREATE_HIP(alloc_zero(&c->s[l].r, n, c->stream));
+    CREATE_HIP(alloc_zero(&c->s[l].cx, n, c->stream));
+    CREATE_HIP(alloc_zero(&c->s[l].cz, nz, c->stream));
+  }
+  const size_t nme = (size_t)(2 * c->nx - 1) * (2 * c->nz - 1);
+  CREATE_HIP(hipMalloc(&c->mu_equi, nme * sizeof(double)));
+  if (mu_equi) {
+    CREATE_HIP(hipMemcpyAsync(c->mu_equi, mu_equi, nme * sizeof(double),
+                              hipMemcpyHostToDevice, c->stream));
+  } else {
+    std::string e;
+    if (build_mu_equi_device(cfg->ni, cfg->nk, mu, c->mu_equi, c->stream, &e) !=
+        QDEQ_OK) {
+      set_error(nullptr, QDEQ_ERR_HIP, "mu_equi: " + e);
+      free_ctx(c);
+      return QDEQ_ERR_HIP;
+    }
+  }
+  CREATE_HIP(alloc_zero(&c->partial, 2 * 1024, c->stream));
+  CREATE_HIP(alloc_zero(&c->scal, SC_COUNT, c->stream));
+  CREATE_HIP(alloc_zero(&c->flags, 4, c->stream));
+  CREATE_HIP(hipHostMalloc(&c->scal_host, SC_COUNT * sizeof(double)));
+  CREATE_HIP(hipHostMalloc(&c->flags_host, 4 * sizeof(int)));
+  CREATE_HIP(alloc_zero(&c->stage_U, (size_t)2 * c->nx * c->nz, c->stream));
+  for (auto &p : c->stage_v) CREATE_HIP(alloc_zero(&p, (size_t)c->nz, c->stream));
+
+  // fault / RK43 buffers: 40 vectors of nz, then 14 device pointers
+  const size_t nzs = (size_t)c->nz;
+  CREATE_HIP(hipMalloc(&c->fb.base, (40 * nzs + 14) * sizeof(double)));
+  CREATE_HIP(hipMemsetAsync(c->fb.base, 0, 40 * nzs * sizeof(double), c->stream));
+  {
+    FaultBufs &f = c->fb;
+    double *b = f.base;
+    double **slots[] = {&f.slip, &f.psi, &f.slip_shift, &f.a, &f.b, &f.eta,
+                        &f.tau_n0, &f.mu_f, &f.Ts, &f.Tn};
+    int q = 0;
+    for (double **s : slots) *s = b + (q++) * nzs;
+    for (int s = 1; s <= 6; s++) f.Ys[s] = b + (q++) * nzs;
+    for (int s = 1; s <= 6; s++) f.Yp[s] = b + (q++) * nzs;
+    for (int s = 1; s <= 6; s++) f.FV[s] = b + (q++) * nzs;
+    for (int s = 1; s <= 6; s++) f.Fd[s] = b + (q++) * nzs;
+    f.y3s = b + (q++) * nzs;
+    f.y3p = b + (q++) * nzs;
+    f.y4s = b + (q++) * nzs;
+    f.y4p = b + (q++) * nzs;
+    f.Ys[0] = f.Yp[0] = f.FV[0] = f.Fd[0] = nullptr;
+    double *ptrs[14];
+    for (int s = 0; s < 7; s++) {
+      ptrs[s] = f.FV[s];
+      ptrs[7 + s] = f.Fd[s];
+    }
+    CREATE_HIP(hipMemcpyAsync(b + 40 * nzs, ptrs, sizeof(ptrs),
+                              hipMemcpyHostToDevice, c->stream));
+    c->mu_fault.assign(nzs, 0.0);
+    for (int k = 1; k <= cfg->nk; k++) c->mu_fault[k] = mu[(size_t)k * c->nx + 1];
+    CREATE_HIP(hipMemcpyAsync(f.mu_f, c->mu_fault.data(), nzs * sizeof(double),
+                              hipMemcpyHostToDevice, c->stream));
+    CREATE_HIP(hipStreamSynchronize(c->stream));
+  }
+  if (mg_build_coefficients(c) != QDEQ_OK) {
+    set_error(nullptr, QDEQ_ERR_HIP, c->err);
+    free_ctx(c);
+    return QDEQ_ERR_HIP;
+  }
+  *out = c;
+  return QDEQ_OK;
+}
+
+int qdeq_destroy(qdeq_ctx *ctx) {
+  free_ctx(ctx);
+  return QDEQ_OK;
+}
+
+int qdeq_get_mu_equi(qdeq_ctx *c, double *me, int where) {
+  if (!c || !me) return set_error(c, QDEQ_ERR_ARG, "null argument");
+  const size_t nme = (size_t)(2 * c->nx - 1) * (2 * c->nz - 1);
+  QDEQ_HIP(c, hipMemcpyAsync(me, c->mu_equi, nme * sizeof(double),
+                             where == QDEQ_HOST ? hipMemcpyDeviceToHost
+                                                : hipMemcpyDeviceToDevice,
+                             c->stream));
+  QDEQ_HIP(c, hipStreamSynchronize(c->stream));
+  return QDEQ_OK;
+}
+
+int qdeq_build_mu_equi(int ni, int nk, const double *mu, double *me,
+                       int device) {
+  if (ni < 3 || nk < 3 || !mu || !me)
+    return set_error(nullptr, QDEQ_ERR_ARG, "qdeq_build_mu_equi: bad argument");
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess)
+    return set_error(nullptr, QDEQ_ERR_HIP, hipGetErrorString(e));
+  const size_t nme = (size_t)(2 * (ni + 2) - 1) * (2 * (nk + 2) - 1);
+  double *d = nullptr;
+  hipStream_t s = nullptr;
+  e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&d, nme * sizeof(double));
+  if (e != hipSuccess) {
+    if (s) (void)hipStreamDestroy(s);
+    return set_error(nullptr, QDEQ_ERR_HIP, hipGetErrorString(e));
+  }
+  std::string msg;
+  int st = build_mu_equi_device(ni, nk, mu, d, s, &msg);
+  if (st == QDEQ_OK) {
+    e = hipMemcpyAsync(me, d, nme * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      msg = hipGetErrorString(e);
+      st = QDEQ_ERR_HIP;
+    }
+  }
+  (void)hipFree(d);
+  (void)hipStreamDestroy(s);
+  if (st != QDEQ_OK) return set_error(nullptr, st, msg);
+  return QDEQ_OK;
+}
+
+// ---- drop-in entry points ----
+
+static int to_device(qdeq_ctx *c, const double *src, double *stage, size_t n,
+                     int where, const double **dst) {
+  if (where == QDEQ_DEVICE) {
+    *dst = src;
+    return QDEQ_OK;
+  }
+  QDEQ_HIP(c, hipMemcpyAsync(stage, src, n * sizeof(double),
+                             hipMemcpyHostToDevice, c->stream));
+  *dst = stage;
+  return QDEQ_OK;
+}
+
+static int to_host(qdeq_ctx *c, double *dst, const double *stage, size_t n,
+                   int where) {
+  if (where == QDEQ_DEVICE) return QDEQ_OK;
+  QDEQ_HIP(c, hipMemcpyAsync(dst, stage, n * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+  return QDEQ_OK;
+}
+
+int qdeq_ode_solve(qdeq_ctx *c, double *U, int where, int *vcycles,
+                   double *res0, double *res) {
+  if (!c || !U) return set_error(c, QDEQ_ERR_ARG, "null argument");
+  const size_t ng = (size_t)c->nx * c->nz;
+  const double *Ud;
+  int st = to_device(c, U, c->stage_U, ng, where, &Ud);
+  if (st) return st;
+  double *Uw = const_cast<double *>(Ud);
+  launch_pack(c, Uw);
+  st = mg_solve(c, vcycles, res0, res);
+  launch_unpack(c, Uw);
+  launch_extend_crew(c, Uw);
+  QDEQ_HIP(c, hipGetLastError());
+  int st2 = to_host(c, U, c->stage_U, ng, where);
+  QDEQ_HIP(c, hipStreamSynchronize(c->stream));
+  return st ? st : st2;
+}
+
+int qdeq_set_boundary(qdeq_ctx *c, double *U, const double *slip,
+                      const double *ss, double t, double ratio, double dt,
+                      int where) {
+  if (!c || !U || !slip || !ss) return set_error(c, QDEQ_ERR_ARG, "null argument");
+  const size_t ng = (size_t)c->nx * c->nz, nz = (size_t)c->nz;
+  const double *Ud, *sd, *ssd;
+  int st;
+  if ((st = to_device(c, U, c->stage_U, ng, where, &Ud))) return st;
+  if ((st = to_device(c, slip, c->stage_v[0], nz, where, &sd))) return st;
+  if ((st = to_device(c, ss, c->stage_v[1], nz, where, &ssd))) return st;
+  launch_set_boundary_grid(c, const_cast<double *>(Ud), sd, ssd, t, ratio, dt);
+  QDEQ_HIP(c, hipGetLastError());
+  if ((st = to_host(c, U, c->stage_U, ng, where))) return st;
+  QDEQ_HIP(c, hipStreamSynchronize(c->stream));
+  return QDEQ_OK;
+}
+
+int qdeq_eval_stress(qdeq_ctx *c, const double *U, const double *mu,
+                     const double *tau_n0, double *Ts, double *Tn, int where) {
+  if (!c || !U || !mu || !tau_n0 || !Ts || !Tn)
+    return set_error(c, QDEQ_ERR_ARG, "null argument");
+  const size_t ng = (size_t)c->nx * c->nz, nz = (size_t)c->nz;
+  const double *Ud, *mud, *td;
+  int st;
+  if ((st = to_device(c, U, c->stage_U, ng, where, &Ud))) return st;
+  if ((st = to_device(c, mu, c->stage_U + ng, ng, where, &mud))) return st;
+  if ((st = to_device(c, tau_n0, c->stage_v[0], nz, where, &td))) return st;
+  double *Tsd = where == QDEQ_DEVICE ? Ts : c->stage_v[1];
+  double *Tnd = where == QDEQ_DEVICE ? Tn : c->stage_v[2];
+  if (where != QDEQ_DEVICE) {
+    QDEQ_HIP(c, hipMemcpyAsync(Tsd, Ts, nz * sizeof(double),
+                               hipMemcpyHostToDevice, c->stream));
+    QDEQ_HIP(c, hipMemcpyAsync(Tnd, Tn, nz * sizeof(double),
+                               hipMemcpyHostToDevice, c->stream));
+  }
+  launch_eval_stress_grid(c, Ud, mud, td, Tsd, Tnd);
+  QDEQ_HIP(c, hipGetLastError());
+  if ((st = to_host(c, Ts, Tsd, nz, where))) return st;
+  if ((st = to_host(c, Tn, Tnd, nz, where))) return st;
+  QDEQ_HIP(c, hipStreamSynchronize(c->stream));
+  return QDEQ_OK;
+}
+
+int qdeq_nr_search(qdeq_ctx *c, const double *Ts, const double *Tn,
+                   const double *psi, const double *a, const double *eta,
+                   double *V, int where) {
+  if (!c || !Ts || !Tn || !psi || !a || !eta || !V)
+    return set_error(c, QDEQ_ERR_ARG, "null argument");
+  const size_t nz = (size_t)c->nz;
+  const double *p[5];
+  const double *src[5] = {Ts, Tn, psi, a, eta};
+  int st;
+  for (int i = 0; i < 5; i++)
+    if ((st = to_device(c, src[i], c->stage_v[i], nz, where, &p[i]))) return st;
+  const double *Vd0;
+  if ((st = to_device(c, V, c->stage_v[5], nz, where, &Vd0))) return st;
+  double *Vd = const_cast<double *>(Vd0);
+  QDEQ_HIP(c, hipMemsetAsync(c->flags, 0, sizeof(int), c->stream));
+  launch_nr(c, p[0], p[1], p[2], p[3], p[4], Vd);
+  QDEQ_HIP(c, hipGetLastError());
+  if ((st = to_host(c, V, Vd, nz, where))) return st;
+  QDEQ_HIP(c, hipMemcpyAsync(c->flags_host, c->flags, sizeof(int),
+                             hipMemcpyDeviceToHost, c->stream));
+  QDEQ_HIP(c, hipStreamSynchronize(c->stream));
+  if (c->flags_host[0])
+    return set_error(c, QDEQ_ERR_NAN_FRICTION, "NaN, error for friction");
+  return QDEQ_OK;
+}
+
+int qdeq_cal_dpsi(qdeq_ctx *c, const double *psi, const double *V,
+                  const double *b, double *dpsi, int where) {
+  if (!c || !psi || !V || !b || !dpsi)
+    return set_error(c, QDEQ_ERR_ARG, "null argument");
+  const size_t nz = (size_t)c->nz;
+  const double *pd, *vd, *bd;
+  int st;
+  if ((st = to_device(c, psi, c->stage_v[0], nz, where, &pd))) return st;
+  if ((st = to_device(c, V, c->stage_v[1], nz, where, &vd))) return st;
+  if ((st = to_device(c, b, c->stage_v[2], nz, where, &bd))) return st;
+  double *dd = where == QDEQ_DEVICE ? dpsi : c->stage_v[3];
+  if (where != QDEQ_DEVICE)
+    QDEQ_HIP(c, hipMemcpyAsync(dd, dpsi, nz * sizeof(double),
+                               hipMemcpyHostToDevice, c->stream));
+  launch_dpsi(c, pd, vd, bd, dd);
+  QDEQ_HIP(c, hipGetLastError());
+  if ((st = to_host(c, dpsi, dd, nz, where))) return st;
+  QDEQ_HIP(c, hipStreamSynchronize(c->stream));
+  return QDEQ_OK;
+}
+
+double qdeq_compute_error(int nk, const double *slip3, const double *psi3,
+                          const double *slip4, const double *psi4) {
+  double err1 = 0.0, err2 = 0.0;
+  for (int k = 1; k <= nk; k++) {
+    const double t1 = slip3[k] - slip4[k];
+    const double t2 = psi3[k] - psi4[k];
+    err1 = err1 + t1 * t1;
+    err2 = err2 + t2 * t2;
+  }
+  err1 = std::sqrt(err1);
+  err2 = std::sqrt(err2);
+  return err1 / (std::sqrt((double)nk) * 1.0) + err2 / (std::sqrt((double)nk) * 1.0);
+}
+
+double qdeq_compute_step_size(int it, double dt, double mindt, double maxdt,
+                              double totErr, double totTol,
+                              const double *errA) {
+  const double ord = 4.0, kappa = 0.9;
+  const double alpha = 0.49 / ord, beta = 0.34 / ord, gamma = 0.1 / ord;
+  double r;
+  if (it < 4)
+    r = kappa * std::pow(totTol / totErr, 1.0 / (1.0 + ord));
+  else
+    r = kappa * std::pow(totTol / totErr, alpha) * std::pow(errA[0] / totTol, beta) *
+        std::pow(totTol / errA[1], gamma);
+  double dT = r * dt;
+  dT = (dt * 5.0 <= dT) ? dt * 5.0 : dT;
+  dT = (maxdt <= dT) ? maxdt : dT;
+  dT = (mindt >= dT) ? mindt : dT;
+  return dT;
+}
+
+// ---- RK43 driver ----
+
+static int check_flags(qdeq_ctx *c) {
+  if (c->flags_host[0])
+    return set_error(c, QDEQ_ERR_NAN_FRICTION, "NaN, error for friction");
+  return QDEQ_OK;
+}
+
+// One stage: boundary data, elliptic solve, fault update.
+static int run_stage(qdeq_ctx *c, int s) {
+  launch_stage(c, s, c->t, c->dt);
+  int st = mg_solve(c, nullptr, nullptr, nullptr);
+  if (st != QDEQ_OK) return st;
+  launch_fault_update(c, s);
+  return QDEQ_OK;
+}
+
+int qdeq_rk43_setup(qdeq_ctx *c, const double *a, const double *b,
+                    const double *eta, const double *tau_n0,
+                    const double *U0) {
+  if (!c || !a || !b || !eta || !tau_n0 || !U0)
+    return set_error(c, QDEQ_ERR_ARG, "null argument");
+  const int nk = c->cfg.nk, ni = c->cfg.ni;
+  const size_t nz = (size_t)c->nz, nx = (size_t)c->nx;
+  FaultBufs &f = c->fb;
+  // fault_init (fault.cpp:116-133)
+  std::vector<double> slip(nz, 0.0), psi(nz, 0.0), ss(nz, 0.0);
+  for (int k = 1; k <= nk; k++) {
+    psi[k] = std::log(c->cfg.V0 / c->cfg.Vini);
+    psi[k] = c->cfg.f0 + b[k] * psi[k];
+  }
+  for (int k = 1; k <= nk; k++) slip[k] = 2.0 * U0[k * nx + 1];
+  for (size_t k = 0; k < nz; k++) ss[k] = U0[k * nx + ni];
+  QDEQ_HIP(c, hipMemsetAsync(f.base, 0, 40 * nz * sizeof(double), c->stream));
+  struct {
+    double *dst;
+    const double *src;
+  } up[] = {{f.a, a},       {f.b, b},         {f.eta, eta}, {f.tau_n0, tau_n0},
+            {f.slip, slip.data()}, {f.psi, psi.data()}, {f.slip_shift, ss.data()}};
+  for (auto &u : up)
+    QDEQ_HIP(c, hipMemcpyAsync(u.dst, u.src, nz * sizeof(double),
+                               hipMemcpyHostToDevice, c->stream));
+  QDEQ_HIP(c, hipMemcpyAsync(f.mu_f, c->mu_fault.data(), nz * sizeof(double),
+                             hipMemcpyHostToDevice, c->stream));
+  QDEQ_HIP(c, hipMemcpyAsync(c->stage_U, U0, nx * nz * sizeof(double),
+                             hipMemcpyHostToDevice, c->stream));
+  launch_pack(c, c->stage_U);
+  QDEQ_HIP(c, hipMemsetAsync(c->flags, 0, sizeof(int), c->stream));
+  QDEQ_HIP(c, hipGetLastError());
+  c->t = 0.0;
+  c->dt = c->cfg.ini_dt;  // odeSolver.cpp:1363
+  c->it = 0;
+  c->attempts = 0;
+  c->errA[0] = c->errA[1] = 0.0;
+  c->totErr = 0.0;
+  c->vcycles = c->solves = c->ir_sweeps = 0;
+  c->rk_ready = false;
+  return QDEQ_OK;
+}
+
+static int rk43_first_stage(qdeq_ctx *c) {
+  // odeSolver.cpp:1428-1437: stage 1 at t = 0 from y_n.
+  int st = run_stage(c, 1);
+  if (st != QDEQ_OK) return st;
+  QDEQ_HIP(c, hipMemcpyAsync(c->flags_host, c->flags, sizeof(int),
+                             hipMemcpyDeviceToHost, c->stream));
+  QDEQ_HIP(c, hipStreamSynchronize(c->stream));
+  if ((st = check_flags(c))) return st;
+  c->rk_ready = true;
+  return QDEQ_OK;
+}
+
+int qdeq_rk43_step(qdeq_ctx *c, int nsteps, qdeq_step_info *info) {
+  if (!c) return set_error(c, QDEQ_ERR_ARG, "null argument");
+  int st;
+  if (!c->rk_ready) {
+    if ((st = rk43_first_stage(c))) return st;
+  }
+  const qdeq_config &p = c->cfg;
+  for (int n = 0; n < nsteps; n++) {
+    c->it = c->it + 1;
+    int attemptCount = 0;
+    while (attemptCount < 100) {  // odeSolver.cpp:1466-1604
+      attemptCount = attemptCount + 1;
+      if (c->t + c->dt > p.tmax) c->dt = p.tmax - c->t;
+      launch_zero_stage_rates(c);
+      for (int s = 2; s <= 6; s++)
+        if ((st = run_stage(c, s))) return st;
+      launch_combine(c, c->dt);
+      QDEQ_HIP(c, hipGetLastError());
+      QDEQ_HIP(c, hipMemcpyAsync(c->scal_host + SC_TOTERR, c->scal + SC_TOTERR,
+                                 sizeof(double), hipMemcpyDeviceToHost,
+                                 c->stream));
+      QDEQ_HIP(c, hipMemcpyAsync(c->flags_host, c->flags, sizeof(int),
+                                 hipMemcpyDeviceToHost, c->stream));
+      QDEQ_HIP(c, hipStreamSynchronize(c->stream));
+      if ((st = check_flags(c))) return st;
+      c->totErr = c->scal_host[SC_TOTERR];
+      if (c->totErr < p.totTol) break;
+      c->dt = qdeq_compute_step_size(c->it, c->dt, p.mindt, p.maxdt, c->totErr,
+                                     p.totTol, c->errA);
+      if (p.mindt == c->dt) break;
+    }
+    c->attempts = attemptCount;
+    c->t = c->t + c->dt;  // odeSolver.cpp:1605-1619
+    launch_commit(c);
+    if ((st = run_stage(c, 1))) return st;
+    double newDeltaT;  // odeSolver.cpp:1621-1632
+    if (c->totErr != 0.0)
+      newDeltaT = qdeq_compute_step_size(c->it, c->dt, p.mindt, p.maxdt,
+                                         c->totErr, p.totTol, c->errA);
+    else
+      newDeltaT = c->dt;
+    c->errA[1] = c->errA[0];
+    c->errA[0] = c->totErr;
+    c->dt = newDeltaT;
+  }
+  QDEQ_HIP(c, hipMemcpyAsync(c->flags_host, c->flags, sizeof(int),
+                             hipMemcpyDeviceToHost, c->stream));
+  QDEQ_HIP(c, hipStreamSynchronize(c->stream));
+  if ((st = check_flags(c))) return st;
+  if (info) return qdeq_rk43_get_state(c, info, nullptr, nullptr, nullptr,
+                                       nullptr, nullptr, nullptr);
+  return QDEQ_OK;
+}
+
+int qdeq_rk43_get_state(qdeq_ctx *c, qdeq_step_info *info, double *U,
+                        double *slip, double *psi, double *V, double *dpsi,
+                        double *Ts) {
+  if (!c) return set_error(c, QDEQ_ERR_ARG, "null argument");
+  const size_t nz = (size_t)c->nz, ng = (size_t)c->nx * c->nz;
+  FaultBufs &f = c->fb;
+  if (info) {
+    launch_maxima(c);
+    QDEQ_HIP(c, hipMemcpyAsync(c->scal_host + SC_VMAX, c->scal + SC_VMAX,
+                               2 * sizeof(double), hipMemcpyDeviceToHost,
+                               c->stream));
+  }
+  if (U) {
+    QDEQ_HIP(c, hipMemsetAsync(c->stage_U, 0, ng * sizeof(double), c->stream));
+    launch_unpack(c, c->stage_U);
+    launch_extend_crew(c, c->stage_U);
+    QDEQ_HIP(c, hipMemcpyAsync(U, c->stage_U, ng * sizeof(double),
+                               hipMemcpyDeviceToHost, c->stream));
+  }
+  struct {
+    double *dst;
+    const double *src;
+  } dn[] = {{slip, f.slip}, {psi, f.psi}, {V, f.FV[1]}, {dpsi, f.Fd[1]}, {Ts, f.Ts}};
+  for (auto &d : dn)
+    if (d.dst)
+      QDEQ_HIP(c, hipMemcpyAsync(d.dst, d.src, nz * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream));
+  QDEQ_HIP(c, hipGetLastError());
+  QDEQ_HIP(c, hipStreamSynchronize(c->stream));
+  if (info) {
+    info->t = c->t;
+    info->dt = c->dt;
+    info->totErr = c->totErr;
+    info->errA0 = c->errA[0];
+    info->errA1 = c->errA[1];
+    info->V_max = c->scal_host[SC_VMAX];
+    info->Ts_max = c->scal_host[SC_TSMAX];
+    info->it = c->it;
+    info->attempts = c->attempts;
+    info->vcycles = c->vcycles;
+    info->solves = c->solves;
+    info->ir_sweeps = c->ir_sweeps;
+  }
+  return QDEQ_OK;
+}
+
+}  // extern "C"

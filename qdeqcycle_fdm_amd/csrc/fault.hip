@@ -1,0 +1,427 @@
+// fault.hip -- fault traction, rate-and-state slip rate, state rate and the
+// RK43 stage arithmetic, one thread per fault node, FP64.
+//
+// Follows XuSun-SUSTech/QDEqCycle_FDM fault.cpp:139-232 and
+// odeSolver.cpp:1150-1206, 1491-1614.  Expressions keep the reference's
+// operation order (-ffp-contract=off); exp/sinh/cosh/asinh are the
+// host/device-identical implementations of qd_math.h.
+#include <cmath>
+
+#include "qd_math.h"
+#include "qdeq_internal.h"
+
+namespace qdeq {
+
+constexpr int FT = 256;  // threads per block for per-node kernels
+
+static inline dim3 nodes_grid(int nk) { return dim3((unsigned)((nk + FT - 1) / FT)); }
+
+// Butcher tableau (odeSolver.cpp:1297-1334).
+__constant__ double kA[7][6] = {
+    {0, 0, 0, 0, 0, 0},
+    {0, 0, 0, 0, 0, 0},
+    {0, 1. / 2., 0, 0, 0, 0},
+    {0, 13861. / 62500., 6889. / 62500., 0, 0, 0},
+    {0, -116923316275. / 2393684061468., -2731218467317. / 15368042101831.,
+     9408046702089. / 11113171139209., 0, 0},
+    {0, -451086348788. / 2902428689909., -2682348792572. / 7519795681897.,
+     12662868775082. / 11960479115383., 3355817975965. / 11060851509271., 0},
+    {0, 647845179188. / 3216320057751., 73281519250. / 8382639484533.,
+     552539513391. / 3454668386233., 3354512671639. / 8306763924573.,
+     4040. / 17871.}};
+static const double kC[7] = {0, 0, 1. / 2., 83. / 250., 31. / 50., 17. / 20., 1.};
+constexpr double B1 = 82889. / 524892., B3 = 15625. / 83664.,
+                 B4 = 69875. / 102672., B5 = -2260. / 8211., B6 = 1. / 4.;
+constexpr double HB1 = 4586570599. / 29645900160., HB3 = 178811875. / 945068544.,
+                 HB4 = 814220225. / 1159782912., HB5 = -3700637. / 11593932.,
+                 HB6 = 61727. / 225920.;
+
+// NRsearch for one node (fault.cpp:180-228).  Returns false on NaN.
+__device__ inline bool nr_node(double Ts, double Tn, double psi, double a,
+                               double eta, double V0, double Vprev,
+                               double *Vout) {
+  for (int attempt = 0; attempt < 2; attempt++) {  // the reference retries once
+    int iter = 0;
+    double var_old = 0.5 * Vprev / V0 * qd_exp(psi / a);
+    var_old = qd_asinh(var_old);
+    const double fa = a * fabs(Tn);
+    const double fb = Ts;
+    double var_new = var_old;
+    const int iter_max = 100000;
+    while (iter < iter_max) {
+      const double fy = 2.0 * eta * V0 * qd_exp(-psi / a) * qd_sinh(var_old) +
+                        fa * var_old - fb;
+      const double df = 2.0 * eta * V0 * qd_exp(-psi / a) * qd_cosh(var_old) + fa;
+      const double d = -fy / (df + 1e-100);
+      var_new = var_old + d;
+      const double err = fabs(d) / (fabs(var_old) + 1e-100);
+      iter = iter + 1;
+      var_old = var_new;
+      if (err < 1e-10) break;
+    }
+    const double friction_update = a * var_new;
+    if (!isnan(friction_update)) {
+      double V = 2.0 * V0 * qd_exp(-psi / a);
+      *Vout = V * qd_sinh(var_new);
+      return true;
+    }
+  }
+  return false;
+}
+
+// cal_dpsi for one node (odeSolver.cpp:1198-1203).
+__device__ inline double dpsi_node(double psi, double V, double b, double f0,
+                                   double V0, double L) {
+  const double A = qd_exp((f0 - psi) / b);
+  double d = 0.0;
+  if (!isinf(A) && b > 1e-3) d = (b * V0 / L) * (A - V / V0);
+  return d;
+}
+
+// ---- drop-in kernels on the reference's nx*nz grid array ----
+
+// setBoundary (odeSolver.cpp:1150-1188): boundary columns; ghost rows after.
+__global__ void k_set_boundary_cols(double *U, int ni, int nk,
+                                    const double *slip, const double *ss,
+                                    double Vini, double t, double ratio,
+                                    double dt) {
+  const int nx = ni + 2, nz = nk + 2;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nz) return;
+  if (k >= 1 && k <= nk) U[(size_t)k * nx + 1] = 0.5 * slip[k];
+  U[(size_t)k * nx + ni] = 0.50 * Vini * (t + ratio * dt) + ss[k];
+}
+__global__ void k_set_boundary_rows(double *U, int ni, int nk) {
+  const int nx = ni + 2;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nx) return;
+  U[i] = U[(size_t)2 * nx + i];
+  U[(size_t)(nk + 1) * nx + i] = U[(size_t)(nk - 1) * nx + i];
+}
+
+// extend_crew (odeSolver.cpp:612-650): columns first, then rows.
+__global__ void k_extend_cols(double *U, int ni, int nk) {
+  const int nx = ni + 2, nz = nk + 2;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nz) return;
+  const size_t i1 = 1 + (size_t)k * nx, i2 = ni + (size_t)k * nx;
+  U[i1 - 1] = 2.0 * U[i1] - U[i1 + 1];
+  U[i2 + 1] = 2.0 * U[i2] - U[i2 - 1];
+}
+__global__ void k_extend_rows(double *U, int ni, int nk) {
+  const int nx = ni + 2;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nx) return;
+  U[i] = U[(size_t)2 * nx + i];
+  U[(size_t)(nk + 1) * nx + i] = U[(size_t)(nk - 1) * nx + i];
+}
+
+// pack/unpack between the nx*nz grid array and the compact level-0 array
+// (odeSolver.cpp:669-677, 708-716).
+__global__ void k_pack(const double *U, double *u, int ni, int nk, int P) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;  // compact column
+  const int j = blockIdx.y;                              // compact row
+  if (i >= ni || j >= nk) return;
+  u[(size_t)j * P + i] = U[(size_t)(j + 1) * (ni + 2) + i + 1];
+}
+__global__ void k_unpack(double *U, const double *u, int ni, int nk, int P) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y;
+  if (i >= ni || j >= nk) return;
+  U[(size_t)(j + 1) * (ni + 2) + i + 1] = u[(size_t)j * P + i];
+}
+
+// eval_stress_on_fault + partial1st (fault.cpp:139-164) on a grid array.
+__global__ void k_eval_stress_grid(const double *U, const double *mu,
+                                   const double *tau_n0, int ni, int nk,
+                                   double c0, double c1, double c2, double c3,
+                                   double *Ts, double *Tn) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  if (k > nk) return;
+  const size_t index = 1 + (size_t)k * (ni + 2);
+  const double du =
+      c0 * U[index] + c1 * U[index + 1] + c2 * U[index + 2] + c3 * U[index + 3];
+  Ts[k] = mu[index] * du * 1.0e3;
+  Tn[k] = tau_n0[k];
+}
+
+__global__ void k_nr(const double *Ts, const double *Tn, const double *psi,
+                     const double *a, const double *eta, double V0, double *V,
+                     int nk, int *flags) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  if (k > nk) return;
+  double Vn;
+  if (nr_node(Ts[k], Tn[k], psi[k], a[k], eta[k], V0, V[k], &Vn))
+    V[k] = Vn;
+  else
+    flags[0] = 1;
+}
+
+__global__ void k_dpsi(const double *psi, const double *V, const double *b,
+                       double f0, double V0, double L, double *dpsi, int nk) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  if (k > nk) return;
+  dpsi[k] = dpsi_node(psi[k], V[k], b[k], f0, V0, L);
+}
+
+// ---- device-resident RK43 stages ----
+
+// Stage state Y_s = y_n + dt sum_j a_sj F_j (odeSolver.cpp:1491-1561) and
+// its boundary data written into the compact level-0 u (setBoundary with
+// ratio c_s: fault column slip/2, remote column Vini (t + c_s dt)/2 +
+// SlipShift).  s = 1 is the committed state (odeSolver.cpp:1615).
+__global__ void k_stage(int s, double dt, double t, double ratio, double Vini,
+                        const double *slip, const double *psi,
+                        const double *ss, double *const *FV,
+                        double *const *Fd, double *Ys, double *Yp, double *u,
+                        int nk, int K, int P) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  if (k > nk) return;
+  double ys = slip[k], yp = psi[k];
+  if (s >= 2) {
+    ys = slip[k] + kA[s][1] * dt * FV[1][k];
+    yp = psi[k] + kA[s][1] * dt * Fd[1][k];
+    for (int jj = 2; jj < s; jj++) {
+      ys = ys + kA[s][jj] * dt * FV[jj][k];
+      yp = yp + kA[s][jj] * dt * Fd[jj][k];
+    }
+  }
+  Ys[k] = ys;
+  Yp[k] = yp;
+  double *row = u + (size_t)(k - 1) * P;
+  row[0] = 0.5 * ys;
+  row[K - 1] = 0.50 * Vini * (t + ratio * dt) + ss[k];
+}
+
+// Fault update after a stage solve: Ts from the compact u (columns 0..3 are
+// the fault and its three neighbours), NR for V from the stage's previous
+// rate (zeroed, odeSolver.cpp:1484-1488), then dpsi.
+__global__ void k_fault_update(const double *u, int P, int nk,
+                               const double *mu_f, const double *tau_n0,
+                               const double *Yp, const double *a,
+                               const double *b, const double *eta, double V0,
+                               double f0, double L, double c0, double c1,
+                               double c2, double c3, double *Ts, double *Tn,
+                               double *V, double *dpsi, int *flags) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  if (k > nk) return;
+  const double *row = u + (size_t)(k - 1) * P;
+  const double du = c0 * row[0] + c1 * row[1] + c2 * row[2] + c3 * row[3];
+  const double ts = mu_f[k] * du * 1.0e3;
+  const double tn = tau_n0[k];
+  Ts[k] = ts;
+  Tn[k] = tn;
+  const double psi = Yp[k];
+  double Vn;
+  if (!nr_node(ts, tn, psi, a[k], eta[k], V0, V[k], &Vn)) {
+    flags[0] = 1;
+    Vn = V[k];
+  }
+  V[k] = Vn;
+  dpsi[k] = dpsi_node(psi, Vn, b[k], f0, V0, L);
+}
+
+__global__ void k_zero_rates(double *const *FV, double *const *Fd, int nz) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nz) return;
+  for (int s = 2; s <= 6; s++) {
+    FV[s][k] = 0.0;
+    Fd[s][k] = 0.0;
+  }
+}
+
+// y3 (embedded, hb weights) and y4 (b weights, propagated) and the error
+// norm (odeSolver.cpp:1569-1595, computeError 1211-1238).  One block; the
+// squared differences are summed by one lane in node order, so totErr has
+// the reference's summation order.
+__global__ __launch_bounds__(1024) void k_combine(
+    double dt, const double *slip, const double *psi, double *const *FV,
+    double *const *Fd, double *y3s, double *y3p, double *y4s, double *y4p,
+    int nk, double *out) {
+  __shared__ double s1[1024], s2[1024];
+  const int tid = threadIdx.x;
+  double e1 = 0.0, e2 = 0.0;  // meaningful in lane 0 only
+  for (int base = 1; base <= nk; base += 1024) {
+    const int k = base + tid;
+    if (k <= nk) {
+      double a3 = slip[k] + HB1 * dt * FV[1][k];
+      a3 = a3 + HB3 * dt * FV[3][k];
+      a3 = a3 + HB4 * dt * FV[4][k];
+      a3 = a3 + HB5 * dt * FV[5][k];
+      a3 = a3 + HB6 * dt * FV[6][k];
+      double p3 = psi[k] + HB1 * dt * Fd[1][k];
+      p3 = p3 + HB3 * dt * Fd[3][k];
+      p3 = p3 + HB4 * dt * Fd[4][k];
+      p3 = p3 + HB5 * dt * Fd[5][k];
+      p3 = p3 + HB6 * dt * Fd[6][k];
+      double a4 = slip[k] + B1 * dt * FV[1][k];
+      a4 = a4 + B3 * dt * FV[3][k];
+      a4 = a4 + B4 * dt * FV[4][k];
+      a4 = a4 + B5 * dt * FV[5][k];
+      a4 = a4 + B6 * dt * FV[6][k];
+      double p4 = psi[k] + B1 * dt * Fd[1][k];
+      p4 = p4 + B3 * dt * Fd[3][k];
+      p4 = p4 + B4 * dt * Fd[4][k];
+      p4 = p4 + B5 * dt * Fd[5][k];
+      p4 = p4 + B6 * dt * Fd[6][k];
+      y3s[k] = a3; y3p[k] = p3; y4s[k] = a4; y4p[k] = p4;
+      const double t1 = a3 - a4, t2 = p3 - p4;
+      s1[tid] = t1 * t1;
+      s2[tid] = t2 * t2;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      const int n = (nk - base + 1) < 1024 ? (nk - base + 1) : 1024;
+      for (int i = 0; i < n; i++) {
+        e1 = e1 + s1[i];
+        e2 = e2 + s2[i];
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const double r1 = sqrt(e1), r2 = sqrt(e2);
+    out[0] = r1 / (sqrt((double)nk) * 1.0) + r2 / (sqrt((double)nk) * 1.0);
+  }
+}
+
+// Accept: y_n <- y4, F_1 <- 0 (odeSolver.cpp:1606-1614).
+__global__ void k_commit(double *slip, double *psi, const double *y4s,
+                         const double *y4p, double *V1, double *d1, int nk) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  if (k > nk) return;
+  slip[k] = y4s[k];
+  psi[k] = y4p[k];
+  V1[k] = 0.0;
+  d1[k] = 0.0;
+}
+
+// maxval (fault.cpp:22-52) of V and Ts.  One block.
+__global__ __launch_bounds__(1024) void k_maxima(const double *V,
+                                                 const double *Ts, int nk,
+                                                 double *out) {
+  __shared__ double s1[1024], s2[1024];
+  double m1 = V[1], m2 = Ts[1];
+  for (int k = threadIdx.x + 1; k <= nk; k += 1024) {
+    m1 = fmax(m1, V[k]);
+    m2 = fmax(m2, Ts[k]);
+  }
+  const int tid = threadIdx.x;
+  s1[tid] = m1;
+  s2[tid] = m2;
+  __syncthreads();
+  for (int st = 512; st > 0; st >>= 1) {
+    if (tid < st) {
+      s1[tid] = fmax(s1[tid], s1[tid + st]);
+      s2[tid] = fmax(s2[tid], s2[tid + st]);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    out[0] = s1[0];
+    out[1] = s2[0];
+  }
+}
+
+// ---- launchers ----
+
+void launch_set_boundary_grid(qdeq_ctx *c, double *U, const double *slip,
+                              const double *ss, double t, double ratio,
+                              double dt) {
+  const int ni = c->cfg.ni, nk = c->cfg.nk;
+  hipLaunchKernelGGL(k_set_boundary_cols, nodes_grid(nk + 2), dim3(FT), 0,
+                     c->stream, U, ni, nk, slip, ss, c->cfg.Vini, t, ratio, dt);
+  hipLaunchKernelGGL(k_set_boundary_rows, nodes_grid(ni + 2), dim3(FT), 0,
+                     c->stream, U, ni, nk);
+}
+
+void launch_extend_crew(qdeq_ctx *c, double *U) {
+  const int ni = c->cfg.ni, nk = c->cfg.nk;
+  hipLaunchKernelGGL(k_extend_cols, nodes_grid(nk + 2), dim3(FT), 0, c->stream,
+                     U, ni, nk);
+  hipLaunchKernelGGL(k_extend_rows, nodes_grid(ni + 2), dim3(FT), 0, c->stream,
+                     U, ni, nk);
+}
+
+void launch_pack(qdeq_ctx *c, const double *U) {
+  hipLaunchKernelGGL(k_pack, dim3((unsigned)((c->cfg.ni + 255) / 256), c->cfg.nk),
+                     dim3(256), 0, c->stream, U, c->d[0].u, c->cfg.ni,
+                     c->cfg.nk, c->lv[0].P);
+}
+
+void launch_unpack(qdeq_ctx *c, double *U) {
+  hipLaunchKernelGGL(k_unpack, dim3((unsigned)((c->cfg.ni + 255) / 256), c->cfg.nk),
+                     dim3(256), 0, c->stream, U, c->d[0].u, c->cfg.ni,
+                     c->cfg.nk, c->lv[0].P);
+}
+
+void launch_eval_stress_grid(qdeq_ctx *c, const double *U, const double *mu,
+                             const double *tau_n0, double *Ts, double *Tn) {
+  hipLaunchKernelGGL(k_eval_stress_grid, nodes_grid(c->cfg.nk), dim3(FT), 0,
+                     c->stream, U, mu, tau_n0, c->cfg.ni, c->cfg.nk,
+                     c->c_stress[0], c->c_stress[1], c->c_stress[2],
+                     c->c_stress[3], Ts, Tn);
+}
+
+void launch_nr(qdeq_ctx *c, const double *Ts, const double *Tn,
+               const double *psi, const double *a, const double *eta,
+               double *V) {
+  hipLaunchKernelGGL(k_nr, nodes_grid(c->cfg.nk), dim3(FT), 0, c->stream, Ts,
+                     Tn, psi, a, eta, c->cfg.V0, V, c->cfg.nk, c->flags);
+}
+
+void launch_dpsi(qdeq_ctx *c, const double *psi, const double *V,
+                 const double *b, double *dpsi) {
+  hipLaunchKernelGGL(k_dpsi, nodes_grid(c->cfg.nk), dim3(FT), 0, c->stream,
+                     psi, V, b, c->cfg.f0, c->cfg.V0, c->cfg.L, dpsi,
+                     c->cfg.nk);
+}
+
+// Device array-of-pointers for the stage rates lives right after the
+// fault buffers (set up in api.hip): fb.base + 40*nz.
+static double *const *dev_FV(qdeq_ctx *c) {
+  return reinterpret_cast<double *const *>(c->fb.base + (size_t)40 * c->nz);
+}
+static double *const *dev_Fd(qdeq_ctx *c) { return dev_FV(c) + 7; }
+
+void launch_stage(qdeq_ctx *c, int s, double t, double dt) {
+  FaultBufs &f = c->fb;
+  hipLaunchKernelGGL(k_stage, nodes_grid(c->cfg.nk), dim3(FT), 0, c->stream, s,
+                     dt, t, kC[s], c->cfg.Vini, f.slip, f.psi, f.slip_shift,
+                     dev_FV(c), dev_Fd(c), f.Ys[s], f.Yp[s], c->d[0].u,
+                     c->cfg.nk, c->lv[0].K, c->lv[0].P);
+}
+
+void launch_fault_update(qdeq_ctx *c, int s) {
+  FaultBufs &f = c->fb;
+  hipLaunchKernelGGL(k_fault_update, nodes_grid(c->cfg.nk), dim3(FT), 0,
+                     c->stream, c->d[0].u, c->lv[0].P, c->cfg.nk, f.mu_f,
+                     f.tau_n0, f.Yp[s], f.a, f.b, f.eta, c->cfg.V0, c->cfg.f0,
+                     c->cfg.L, c->c_stress[0], c->c_stress[1], c->c_stress[2],
+                     c->c_stress[3], f.Ts, f.Tn, f.FV[s], f.Fd[s], c->flags);
+}
+
+void launch_zero_stage_rates(qdeq_ctx *c) {
+  hipLaunchKernelGGL(k_zero_rates, nodes_grid(c->nz), dim3(FT), 0, c->stream,
+                     dev_FV(c), dev_Fd(c), c->nz);
+}
+
+void launch_combine(qdeq_ctx *c, double dt) {
+  FaultBufs &f = c->fb;
+  hipLaunchKernelGGL(k_combine, dim3(1), dim3(1024), 0, c->stream, dt, f.slip,
+                     f.psi, dev_FV(c), dev_Fd(c), f.y3s, f.y3p, f.y4s, f.y4p,
+                     c->cfg.nk, c->scal + SC_TOTERR);
+}
+
+void launch_commit(qdeq_ctx *c) {
+  FaultBufs &f = c->fb;
+  hipLaunchKernelGGL(k_commit, nodes_grid(c->cfg.nk), dim3(FT), 0, c->stream,
+                     f.slip, f.psi, f.y4s, f.y4p, f.FV[1], f.Fd[1], c->cfg.nk);
+}
+
+void launch_maxima(qdeq_ctx *c) {
+  hipLaunchKernelGGL(k_maxima, dim3(1), dim3(1024), 0, c->stream, c->fb.FV[1],
+                     c->fb.Ts, c->cfg.nk, c->scal + SC_VMAX);
+}
+
+}  // namespace qdeq

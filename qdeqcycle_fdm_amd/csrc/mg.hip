@@ -1,0 +1,455 @@
+// mg.hip -- geometric multigrid for div(mu grad u) = 0 on gfx950.
+//
+// Method (XuSun-SUSTech/QDEqCycle_FDM odeSolver.cpp:652-1147): V-cycle with
+// red-black Gauss-Seidel smoothing, full-weighting restriction, bilinear
+// prolongation, level-0 residual norm as the stop test.  The operator on
+// each level is stored as compact face-coefficient arrays (cx: west face,
+// cz: south face) extracted once from the doubled mu_equi array, so the
+// sweep reads 2 coefficient words per point instead of 4 stride-2 gathers.
+// Sums are formed in the reference's order and the build uses
+// -ffp-contract=off, so the FP64 reference cycle reproduces the reference's
+// iterates bit for bit.
+#include <cmath>
+#include <vector>
+
+#include "qdeq_internal.h"
+
+namespace qdeq {
+
+constexpr int BX = 64, BY = 4;          // 256-thread tiles, 64-wide rows
+constexpr int NORM_BLOCKS = 1024, NORM_THREADS = 256;
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// One colour of RBGS (rbgs_kernel<phase>, odeSolver.cpp:757-861).  Threads
+// are indexed over the points of one colour only: point k = 2t + ((j+c)&1),
+// so every lane does work.  The update reads only the other colour.  Top and
+// bottom rows mirror the vertical neighbour (the reference's Neumann rows);
+// the coarse-level coefficient quirk of the reference (one vertical
+// coefficient used twice) is encoded in cz's first and last rows.
+template <typename T, bool HAS_F, bool EXACT>
+__global__ __launch_bounds__(BX *BY) void k_rbgs(
+    T *__restrict__ u, const T *__restrict__ f, const T *__restrict__ cx,
+    const T *__restrict__ cz, int N, int K, int P, T DH2, T relax,
+    int colour) {
+  const int j = blockIdx.y * BY + threadIdx.y;
+  const int t = blockIdx.x * BX + threadIdx.x;
+  if (j >= N) return;
+  const int k = 2 * t + ((j + colour) & 1);
+  if (k < 1 || k > K - 2) return;
+  const int jS = (j == 0) ? 1 : j - 1;
+  const int jN = (j == N - 1) ? N - 2 : j + 1;
+  const size_t c = (size_t)j * P + k;
+  const T m1 = cx[c], m2 = cx[c + 1], m3 = cz[c], m4 = cz[c + P];
+  const T uW = u[c - 1], uE = u[c + 1];
+  const T uS = u[(size_t)jS * P + k], uN = u[(size_t)jN * P + k];
+  T num = m1 * uW + m2 * uE + m3 * uS + m4 * uN;
+  if (HAS_F) num = num - DH2 * f[c];
+  if (EXACT) {
+    u[c] = (relax / (m1 + m2 + m3 + m4)) * num + (T(1) - relax) * u[c];
+  } else {
+    const T unew = num / (m1 + m2 + m3 + m4);
+    u[c] = (relax == T(1)) ? unew : u[c] + relax * (unew - u[c]);
+  }
+}
+
+// r = f - A u on one level (residual_kernel, odeSolver.cpp:863-951).
+template <typename T, bool HAS_F>
+__global__ __launch_bounds__(BX *BY) void k_residual(
+    T *__restrict__ r, const T *__restrict__ u, const T *__restrict__ f,
+    const T *__restrict__ cx, const T *__restrict__ cz, int N, int K, int P,
+    T DH2) {
+  const int j = blockIdx.y * BY + threadIdx.y;
+  const int k = blockIdx.x * BX + threadIdx.x + 1;
+  if (j >= N || k > K - 2) return;
+  const int jS = (j == 0) ? 1 : j - 1;
+  const int jN = (j == N - 1) ? N - 2 : j + 1;
+  const size_t c = (size_t)j * P + k;
+  const T m1 = cx[c], m2 = cx[c + 1], m3 = cz[c], m4 = cz[c + P];
+  const T fv = HAS_F ? f[c] : T(0);
+  r[c] = fv - (m1 * u[c - 1] + m2 * u[c + 1] + m3 * u[(size_t)jS * P + k] +
+               m4 * u[(size_t)jN * P + k] - (m1 + m2 + m3 + m4) * u[c]) /
+                  DH2;
+}
+
+// Full weighting to the coarse level, zeroing the coarse correction
+// (restriction_kernel, odeSolver.cpp:953-1015).  The first/last coarse rows
+// use the single interior fine neighbour row twice.
+template <typename T>
+__global__ __launch_bounds__(BX *BY) void k_restrict(
+    T *__restrict__ r2, T *__restrict__ e2, const T *__restrict__ r, int Nc,
+    int Kc, int Pc, int Pf) {
+  const int j = blockIdx.y * BY + threadIdx.y;
+  const int k = blockIdx.x * BX + threadIdx.x + 1;
+  if (j >= Nc || k > Kc - 2) return;
+  const int jS = (j == 0) ? 1 : 2 * j - 1;
+  const int jN = (j == Nc - 1) ? 2 * j - 1 : 2 * j + 1;
+  const T *rc = r + (size_t)(2 * j) * Pf;
+  const T *rs = r + (size_t)jS * Pf;
+  const T *rn = r + (size_t)jN * Pf;
+  const int kf = 2 * k;
+  const T v = T(0.0625) *
+              (T(4.0) * rc[kf] + T(2.0) * rc[kf - 1] + T(2.0) * rc[kf + 1] +
+               T(2.0) * rs[kf] + T(2.0) * rn[kf] + rs[kf - 1] + rs[kf + 1] +
+               rn[kf - 1] + rn[kf + 1]);
+  r2[(size_t)j * Pc + k] = v;
+  e2[(size_t)j * Pc + k] = T(0);
+}
+
+// u += P e (prolongate_kernel, odeSolver.cpp:1017-1043): one thread per
+// coarse cell writes its 4 fine points.  The reference skips the last fine
+// row (FULL_TOP = false); the fast cycle also corrects it.
+template <typename T, bool FULL_TOP>
+__global__ __launch_bounds__(BX *BY) void k_prolong(
+    T *__restrict__ u, const T *__restrict__ e, int Nc, int Kc, int Pc,
+    int Pf) {
+  const int j = blockIdx.y * BY + threadIdx.y;
+  const int k = blockIdx.x * BX + threadIdx.x;
+  const int jmax = FULL_TOP ? Nc : Nc - 1;
+  if (j >= jmax || k >= Kc - 1) return;
+  const T *ec = e + (size_t)j * Pc + k;
+  T *uf = u + (size_t)(2 * j) * Pf + 2 * k;
+  const T e00 = ec[0], e01 = ec[1];
+  uf[0] += e00;
+  uf[1] += T(0.5) * (e00 + e01);
+  if (!FULL_TOP || j < Nc - 1) {
+    const T e10 = ec[Pc], e11 = ec[Pc + 1];
+    T *uf1 = uf + Pf;
+    uf1[0] += T(0.5) * (e00 + e10);
+    uf1[1] += T(0.25) * (e00 + e01 + e10 + e11);
+  }
+}
+
+__device__ inline double block_sum_256(double v, double *sh) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) sh[tid] += sh[tid + s];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+// Level-0 residual norm, pass 1: fixed-grid partial sums of r^2 (the
+// reference uses calcuDelta_dev + cublasDnrm2, odeSolver.cpp:693-700).
+// Optionally stores r as FP32 (the IR right-hand side) and zeroes the FP32
+// correction.  Deterministic: fixed grid, fixed per-thread order, fixed tree.
+template <bool STORE32>
+__global__ __launch_bounds__(NORM_THREADS) void k_resnorm_partial(
+    const double *__restrict__ u, const double *__restrict__ cx,
+    const double *__restrict__ cz, int N, int K, int P, double DH2,
+    double *__restrict__ partial, float *__restrict__ r32,
+    float *__restrict__ e32) {
+  __shared__ double sh[NORM_THREADS];
+  const int W = K - 2;
+  const long total = (long)N * W;
+  double acc = 0.0;
+  for (long p = (long)blockIdx.x * NORM_THREADS + threadIdx.x; p < total;
+       p += (long)gridDim.x * NORM_THREADS) {
+    const int j = (int)(p / W);
+    const int k = (int)(p - (long)j * W) + 1;
+    const int jS = (j == 0) ? 1 : j - 1;
+    const int jN = (j == N - 1) ? N - 2 : j + 1;
+    const size_t c = (size_t)j * P + k;
+    const double m1 = cx[c], m2 = cx[c + 1], m3 = cz[c], m4 = cz[c + P];
+    const double r =
+        0.0 - (m1 * u[c - 1] + m2 * u[c + 1] + m3 * u[(size_t)jS * P + k] +
+               m4 * u[(size_t)jN * P + k] - (m1 + m2 + m3 + m4) * u[c]) /
+                  DH2;
+    acc += r * r;
+    if (STORE32) {
+      r32[c] = (float)r;
+      e32[c] = 0.0f;
+    }
+  }
+  const double s = block_sum_256(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// Boundary-data residual ||r(u_interior = 0)||^2 partials: only the columns
+// next to the Dirichlet columns are non-zero.  One thread per row.
+__global__ __launch_bounds__(NORM_THREADS) void k_bdres_partial(
+    const double *__restrict__ u, const double *__restrict__ cx, int N, int K,
+    int P, double DH2, double *__restrict__ partial) {
+  __shared__ double sh[NORM_THREADS];
+  double acc = 0.0;
+  for (int j = blockIdx.x * NORM_THREADS + threadIdx.x; j < N;
+       j += gridDim.x * NORM_THREADS) {
+    const size_t row = (size_t)j * P;
+    for (int pass = 0; pass < 2; pass++) {
+      const int k = pass == 0 ? 1 : K - 2;
+      if (pass == 1 && K - 2 == 1) break;
+      const double uW = (k - 1 == 0) ? u[row] : 0.0;
+      const double uE = (k + 1 == K - 1) ? u[row + K - 1] : 0.0;
+      const double m1 = cx[row + k], m2 = cx[row + k + 1];
+      const double r = 0.0 - (m1 * uW + m2 * uE) / DH2;
+      acc += r * r;
+    }
+  }
+  const double s = block_sum_256(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// Pass 2: one block sums the partials in a fixed order, writes sqrt.
+__global__ __launch_bounds__(NORM_THREADS) void k_norm_final(
+    const double *__restrict__ partial, int n, double *__restrict__ out) {
+  __shared__ double sh[NORM_THREADS];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += NORM_THREADS) acc += partial[i];
+  const double s = block_sum_256(acc, sh);
+  if (threadIdx.x == 0) out[0] = sqrt(s);
+}
+
+// IR update u += e (interior columns), FP64 += FP32.
+__global__ __launch_bounds__(BX *BY) void k_add_correction(
+    double *__restrict__ u, const float *__restrict__ e, int N, int K, int P) {
+  const int j = blockIdx.y * BY + threadIdx.y;
+  const int k = blockIdx.x * BX + threadIdx.x + 1;
+  if (j >= N || k > K - 2) return;
+  const size_t c = (size_t)j * P + k;
+  u[c] += (double)e[c];
+}
+
+// Operator coefficients of one level from the doubled mu_equi array.  Level
+// 0 reads the equivalent-medium faces (odeSolver.cpp:785-789); coarse levels
+// read nodal mu at the fine node under each coarse-face midpoint
+// (odeSolver.cpp:823-827).  On coarse levels the first and last rows of cz
+// repeat their neighbour, reproducing the reference's use of one vertical
+// coefficient twice on the Neumann rows (830-847).
+__global__ void k_extract_coef(const double *__restrict__ me, int nx, int N,
+                               int K, int P, int level, int fc,
+                               double *__restrict__ cx,
+                               double *__restrict__ cz) {
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j > N || k >= K) return;
+  const long W2 = 2L * nx - 1;
+  if (j < N && k >= 1) {
+    long ix;
+    if (level == 0)
+      ix = ((long)(k + 1) * 2 - 1) + ((long)(j + 1) * 2) * W2;
+    else
+      ix = ((long)((k * 2 - 1) * fc / 2) + 1) * 2 + ((long)(j * fc + 1) * 2) * W2;
+    cx[(size_t)j * P + k] = me[ix];
+  }
+  if (k >= 1 && k <= K - 2) {
+    long iz;
+    if (level == 0) {
+      iz = (long)(k + 1) * 2 + ((long)(j + 1) * 2 - 1) * W2;
+    } else {
+      const int jj = (j == 0) ? 1 : ((j == N) ? N - 1 : j);
+      iz = (long)(k * fc + 1) * 2 + ((long)((jj * 2 - 1) * fc / 2) + 1) * 2 * W2;
+    }
+    cz[(size_t)j * P + k] = me[iz];
+  }
+}
+
+__global__ void k_to_float(const double *__restrict__ a, float *__restrict__ b,
+                           size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+
+static inline dim3 grid_cols(int ncols, int nrows) {
+  return dim3((unsigned)((ncols + BX - 1) / BX), (unsigned)((nrows + BY - 1) / BY));
+}
+
+template <typename T, bool EXACT>
+static void smooth(qdeq_ctx *c, LevelBufs<T> &B, const LevelDims &d, int sweeps,
+                   bool has_f) {
+  const dim3 blk(BX, BY);
+  const dim3 grd = grid_cols((d.K + 1) / 2, d.N);
+  const T DH2 = (T)(d.DH * d.DH);
+  const T relax = (T)c->cfg.relax;
+  for (int it = 0; it < sweeps; it++) {
+    for (int col = 0; col < 2; col++) {
+      if (has_f)
+        hipLaunchKernelGGL((k_rbgs<T, true, EXACT>), grd, blk, 0, c->stream,
+                           B.u, B.f, B.cx, B.cz, d.N, d.K, d.P, DH2, relax, col);
+      else
+        hipLaunchKernelGGL((k_rbgs<T, false, EXACT>), grd, blk, 0, c->stream,
+                           B.u, B.f, B.cx, B.cz, d.N, d.K, d.P, DH2, relax, col);
+    }
+  }
+}
+
+// vcycle (odeSolver.cpp:1101-1147) on buffers B (FP64 or FP32 hierarchy).
+template <typename T, bool EXACT>
+static void vcycle(qdeq_ctx *c, LevelBufs<T> *B, int level, bool level0_has_f) {
+  const LevelDims &d = c->lv[level];
+  const bool ref = c->cfg.cycle == QDEQ_CYCLE_REFERENCE;
+  const bool has_f = level > 0 || level0_has_f;
+  if (level >= c->L - 1) {
+    smooth<T, EXACT>(c, B[level], d, c->cfg.n_coarse, has_f);
+    return;
+  }
+  smooth<T, EXACT>(c, B[level], d, c->cfg.n_pre, has_f);
+  const dim3 blk(BX, BY);
+  const T DH2 = (T)(d.DH * d.DH);
+  if (has_f)
+    hipLaunchKernelGGL((k_residual<T, true>), grid_cols(d.K - 2, d.N), blk, 0,
+                       c->stream, B[level].r, B[level].u, B[level].f,
+                       B[level].cx, B[level].cz, d.N, d.K, d.P, DH2);
+  else
+    hipLaunchKernelGGL((k_residual<T, false>), grid_cols(d.K - 2, d.N), blk, 0,
+                       c->stream, B[level].r, B[level].u, B[level].f,
+                       B[level].cx, B[level].cz, d.N, d.K, d.P, DH2);
+  const LevelDims &dc = c->lv[level + 1];
+  hipLaunchKernelGGL((k_restrict<T>), grid_cols(dc.K - 2, dc.N), blk, 0,
+                     c->stream, B[level + 1].f, B[level + 1].u, B[level].r,
+                     dc.N, dc.K, dc.P, d.P);
+  vcycle<T, EXACT>(c, B, level + 1, level0_has_f);
+  if (ref)
+    hipLaunchKernelGGL((k_prolong<T, false>), grid_cols(dc.K - 1, dc.N - 1),
+                       blk, 0, c->stream, B[level].u, B[level + 1].u, dc.N,
+                       dc.K, dc.P, d.P);
+  else
+    hipLaunchKernelGGL((k_prolong<T, true>), grid_cols(dc.K - 1, dc.N), blk, 0,
+                       c->stream, B[level].u, B[level + 1].u, dc.N, dc.K, dc.P,
+                       d.P);
+  smooth<T, EXACT>(c, B[level], d, c->cfg.n_post, has_f);
+}
+
+// Level-0 residual norm into scal[SC_NORM] (and r32/e32 for IR).
+static void launch_resnorm(qdeq_ctx *c, bool store32) {
+  const LevelDims &d = c->lv[0];
+  const double DH2 = d.DH * d.DH;
+  if (store32)
+    hipLaunchKernelGGL((k_resnorm_partial<true>), dim3(NORM_BLOCKS),
+                       dim3(NORM_THREADS), 0, c->stream, c->d[0].u, c->d[0].cx,
+                       c->d[0].cz, d.N, d.K, d.P, DH2, c->partial, c->s[0].f,
+                       c->s[0].u);
+  else
+    hipLaunchKernelGGL((k_resnorm_partial<false>), dim3(NORM_BLOCKS),
+                       dim3(NORM_THREADS), 0, c->stream, c->d[0].u, c->d[0].cx,
+                       c->d[0].cz, d.N, d.K, d.P, DH2, c->partial,
+                       (float *)nullptr, (float *)nullptr);
+  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(NORM_THREADS), 0, c->stream,
+                     c->partial, NORM_BLOCKS, c->scal + SC_NORM);
+}
+
+static void launch_bdres(qdeq_ctx *c) {
+  const LevelDims &d = c->lv[0];
+  const int nb = 8;
+  hipLaunchKernelGGL(k_bdres_partial, dim3(nb), dim3(NORM_THREADS), 0,
+                     c->stream, c->d[0].u, c->d[0].cx, d.N, d.K, d.P,
+                     d.DH * d.DH, c->partial + NORM_BLOCKS);
+  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(NORM_THREADS), 0, c->stream,
+                     c->partial + NORM_BLOCKS, nb, c->scal + SC_NORM0);
+}
+
+// The launch sequence of one solver iteration, followed by the residual norm.
+static void launch_iteration(qdeq_ctx *c) {
+  if (c->cfg.precision == QDEQ_PREC_FP32_IR) {
+    vcycle<float, false>(c, c->s, 0, true);
+    const LevelDims &d = c->lv[0];
+    hipLaunchKernelGGL(k_add_correction, grid_cols(d.K - 2, d.N), dim3(BX, BY),
+                       0, c->stream, c->d[0].u, c->s[0].u, d.N, d.K, d.P);
+    launch_resnorm(c, true);
+  } else {
+    if (c->cfg.cycle == QDEQ_CYCLE_REFERENCE)
+      vcycle<double, true>(c, c->d, 0, false);
+    else
+      vcycle<double, false>(c, c->d, 0, false);
+    launch_resnorm(c, false);
+  }
+}
+
+void mg_destroy_graphs(qdeq_ctx *c) {
+  if (c->g_cycle) (void)hipGraphExecDestroy(c->g_cycle);
+  c->g_cycle = nullptr;
+  c->g_cycle_mode = -1;
+}
+
+static int ensure_graph(qdeq_ctx *c) {
+  const int mode = c->cfg.precision * 2 + c->cfg.cycle;
+  if (c->g_cycle && c->g_cycle_mode == mode) return QDEQ_OK;
+  mg_destroy_graphs(c);
+  hipGraph_t g = nullptr;
+  QDEQ_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  launch_iteration(c);
+  QDEQ_HIP(c, hipStreamEndCapture(c->stream, &g));
+  hipError_t e = hipGraphInstantiate(&c->g_cycle, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess)
+    return set_error(c, QDEQ_ERR_HIP,
+                     std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  c->g_cycle_mode = mode;
+  return QDEQ_OK;
+}
+
+// multigrid (odeSolver.cpp:652-720) on the device-resident level-0 u whose
+// Dirichlet columns are already set.  Iterates while ||r||_2 > threshold.
+int mg_solve(qdeq_ctx *c, int *vcycles, double *res0, double *res) {
+  const bool ir = c->cfg.precision == QDEQ_PREC_FP32_IR;
+  int st = ensure_graph(c);
+  if (st != QDEQ_OK) return st;
+  if (c->cfg.tol_mode == QDEQ_TOL_REL) launch_bdres(c);
+  launch_resnorm(c, ir);
+  QDEQ_HIP(c, hipGetLastError());
+  QDEQ_HIP(c, hipMemcpyAsync(c->scal_host, c->scal, 2 * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+  QDEQ_HIP(c, hipStreamSynchronize(c->stream));
+  double delta = c->scal_host[SC_NORM];
+  const double r0 =
+      c->cfg.tol_mode == QDEQ_TOL_REL ? c->scal_host[SC_NORM0] : delta;
+  const double thr =
+      c->cfg.tol_mode == QDEQ_TOL_REL ? c->cfg.tol * r0 : c->cfg.tol;
+  int it = 0;
+  st = QDEQ_OK;
+  while (delta > thr) {
+    if (it >= c->cfg.max_vcycles) {
+      st = set_error(c, QDEQ_ERR_MG_NOT_CONVERGED,
+                     "multigrid: V-cycle cap reached (residual " +
+                         std::to_string(delta) + " > " + std::to_string(thr) +
+                         ")");
+      break;
+    }
+    QDEQ_HIP(c, hipGraphLaunch(c->g_cycle, c->stream));
+    QDEQ_HIP(c, hipMemcpyAsync(c->scal_host, c->scal, sizeof(double),
+                               hipMemcpyDeviceToHost, c->stream));
+    QDEQ_HIP(c, hipStreamSynchronize(c->stream));
+    delta = c->scal_host[SC_NORM];
+    it++;
+    if (!std::isfinite(delta)) {
+      st = set_error(c, QDEQ_ERR_MG_NOT_CONVERGED,
+                     "multigrid: residual is not finite");
+      break;
+    }
+  }
+  c->vcycles += it;
+  if (ir) c->ir_sweeps += it;
+  c->solves += 1;
+  if (vcycles) *vcycles = it;
+  if (res0) *res0 = r0;
+  if (res) *res = delta;
+  return st;
+}
+
+// Extracts every level's coefficients (FP64, and FP32 copies for IR).
+int mg_build_coefficients(qdeq_ctx *c) {
+  for (int l = 0; l < c->L; l++) {
+    const LevelDims &d = c->lv[l];
+    const dim3 blk(64, 4);
+    const dim3 grd((d.K + 63) / 64, (d.N + 1 + 3) / 4);
+    hipLaunchKernelGGL(k_extract_coef, grd, blk, 0, c->stream, c->mu_equi,
+                       c->nx, d.N, d.K, d.P, l, d.fc, c->d[l].cx, c->d[l].cz);
+    const size_t ncx = (size_t)d.N * d.P, ncz = (size_t)(d.N + 1) * d.P;
+    hipLaunchKernelGGL(k_to_float, dim3((unsigned)((ncx + 255) / 256)), dim3(256),
+                       0, c->stream, c->d[l].cx, c->s[l].cx, ncx);
+    hipLaunchKernelGGL(k_to_float, dim3((unsigned)((ncz + 255) / 256)), dim3(256),
+                       0, c->stream, c->d[l].cz, c->s[l].cz, ncz);
+  }
+  QDEQ_HIP(c, hipGetLastError());
+  QDEQ_HIP(c, hipStreamSynchronize(c->stream));
+  return QDEQ_OK;
+}
+
+}  // namespace qdeq

@@ -1,0 +1,129 @@
+// qdeq_internal.h -- shared declarations of the HIP engine (not part of the
+// public C ABI, which is include/qdeq.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "qdeq.h"
+
+#define QDEQ_MAX_LEVELS 12
+
+namespace qdeq {
+
+// One multigrid level, compact row-major storage with a padded row pitch P
+// (elements).  Row j is fault-parallel depth row j (0 = bottom), column k is
+// the distance from the fault (0 = fault, K-1 = remote edge).  The level
+// hierarchy is N <- (N-1)/2+1, DH <- 2 DH (odeSolver.cpp:1391-1405).
+struct LevelDims {
+  int N, K, P;   // rows, columns, row pitch in elements
+  int fc;        // 2^level: fine nodes per coarse spacing
+  double DH;     // grid spacing of the level
+};
+
+// Device buffers of one level.  u: level 0 = displacement (FP64 path) or the
+// IR correction (FP32 path); levels >= 1 = coarse correction.  f: right-hand
+// side (level 0 FP64 = zero, never read).  r: residual scratch.  cx: west-face
+// coefficient of point (j,k) (east face = cx[k+1]); cz: south-face coefficient
+// of row j, N+1 rows (north face = row j+1).  Boundary columns of u are
+// Dirichlet data (level 0) or zero (corrections).
+template <typename T>
+struct LevelBufs {
+  T *u = nullptr, *f = nullptr, *r = nullptr, *cx = nullptr, *cz = nullptr;
+};
+
+// RK43 / fault state on the device, all length nz (valid rows 1..nk).
+struct FaultBufs {
+  double *slip, *psi, *slip_shift, *a, *b, *eta, *tau_n0, *mu_f, *Ts, *Tn;
+  double *Ys[7], *Yp[7];   // stage states k1..k6 (slip, psi)
+  double *FV[7], *Fd[7];   // stage rates f1..f6 (V, dpsi)
+  double *y3s, *y3p, *y4s, *y4p;
+  double *base;            // single allocation backing all of the above
+};
+
+// Device scalars (one allocation).
+enum {
+  SC_NORM = 0,    // residual norm of the last check
+  SC_NORM0 = 1,   // boundary-data residual norm
+  SC_TOTERR = 2,
+  SC_VMAX = 3,
+  SC_TSMAX = 4,
+  SC_COUNT = 8
+};
+
+}  // namespace qdeq
+
+struct qdeq_ctx {
+  qdeq_config cfg;
+  int nx, nz, L;
+  qdeq::LevelDims lv[QDEQ_MAX_LEVELS];
+  qdeq::LevelBufs<double> d[QDEQ_MAX_LEVELS];
+  qdeq::LevelBufs<float> s[QDEQ_MAX_LEVELS];
+  double *mu_equi = nullptr;   // doubled equivalent-medium array (device)
+  double *partial = nullptr;   // reduction partials (2 x NORM_BLOCKS)
+  double *scal = nullptr;      // qdeq::SC_* device scalars
+  double *scal_host = nullptr; // pinned mirror of scal
+  int *flags = nullptr;        // [0] NR NaN flag
+  int *flags_host = nullptr;
+  double *stage_U = nullptr;   // nx*nz staging for drop-in entry points
+  double *stage_v[8] = {};     // nz staging vectors
+  qdeq::FaultBufs fb{};
+  hipStream_t stream = nullptr;
+  hipGraphExec_t g_cycle = nullptr;   // one V-cycle (or IR sweep) + residual norm
+  int g_cycle_mode = -1;
+  std::string err;
+  // RK43 host state (odeSolver.cpp:1338-1363)
+  bool rk_ready = false;
+  double t = 0, dt = 0, totErr = 0, errA[2] = {0, 0};
+  int it = 0, attempts = 0;
+  long long vcycles = 0, solves = 0, ir_sweeps = 0;
+  double c_stress[4];  // partial1st coefficients (fault.cpp:162)
+  std::vector<double> mu_fault;  // nodal mu at the fault column (host copy)
+};
+
+namespace qdeq {
+
+int set_error(qdeq_ctx *c, int code, const std::string &msg);
+
+// multigrid (mg.hip)
+int mg_solve(qdeq_ctx *c, int *vcycles, double *res0, double *res);
+int mg_build_coefficients(qdeq_ctx *c);
+void mg_destroy_graphs(qdeq_ctx *c);
+
+// medium (medium.hip)
+int build_mu_equi_device(int ni, int nk, const double *mu_dev, double *me_dev,
+                         hipStream_t s, std::string *err);
+
+// fault + RK43 stages (fault.hip)
+void launch_set_boundary_grid(qdeq_ctx *c, double *U, const double *slip,
+                              const double *slip_shift, double t, double ratio,
+                              double dt);
+void launch_extend_crew(qdeq_ctx *c, double *U);
+void launch_pack(qdeq_ctx *c, const double *U);
+void launch_unpack(qdeq_ctx *c, double *U);
+void launch_eval_stress_grid(qdeq_ctx *c, const double *U, const double *mu,
+                             const double *tau_n0, double *Ts, double *Tn);
+void launch_nr(qdeq_ctx *c, const double *Ts, const double *Tn,
+               const double *psi, const double *a, const double *eta,
+               double *V);
+void launch_dpsi(qdeq_ctx *c, const double *psi, const double *V,
+                 const double *b, double *dpsi);
+void launch_stage(qdeq_ctx *c, int s, double t, double dt);
+void launch_fault_update(qdeq_ctx *c, int s);
+void launch_zero_stage_rates(qdeq_ctx *c);
+void launch_combine(qdeq_ctx *c, double dt);
+void launch_commit(qdeq_ctx *c);
+void launch_maxima(qdeq_ctx *c);
+
+}  // namespace qdeq
+
+#define QDEQ_HIP(c, call)                                                   \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess)                                                   \
+      return qdeq::set_error((c), QDEQ_ERR_HIP,                             \
+                             std::string(#call) + ": " +                    \
+                                 hipGetErrorString(e_));                    \
+  } while (0)

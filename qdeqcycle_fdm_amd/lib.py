@@ -1,0 +1,304 @@
+"""ctypes bindings of libqdeq.so (include/qdeq.h) and a small host-side
+mirror of the reference's call surface.
+
+The reference is one C++ executable whose hot path is
+``RK43 -> setBoundary -> ode_solve -> eval_stress_on_fault -> NRsearch ->
+cal_dpsi`` (odeSolver.cpp:1277-1674).  :class:`Engine` exposes the same
+functions (names and argument meaning kept) over the HIP library, plus the
+device-resident RK43 driver.  There is no CPU fallback: if libqdeq.so is
+missing or no GPU is visible, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import config as cfgmod
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libqdeq.so")
+
+QDEQ_OK = 0
+QDEQ_ERR_ARG = 1
+QDEQ_ERR_HIP = 2
+QDEQ_ERR_NAN_FRICTION = 3
+QDEQ_ERR_MG_NOT_CONVERGED = 4
+QDEQ_ERR_STATE = 5
+QDEQ_ERR_NOMEM = 6
+QDEQ_HOST, QDEQ_DEVICE = 0, 1
+
+# Every entry point include/qdeq.h declares.
+EXPORTS = (
+    "qdeq_default_config", "qdeq_create", "qdeq_destroy", "qdeq_last_error",
+    "qdeq_global_error", "qdeq_get_mu_equi", "qdeq_build_mu_equi",
+    "qdeq_ode_solve", "qdeq_set_boundary", "qdeq_eval_stress",
+    "qdeq_nr_search", "qdeq_cal_dpsi", "qdeq_compute_error",
+    "qdeq_compute_step_size", "qdeq_rk43_setup", "qdeq_rk43_step",
+    "qdeq_rk43_get_state",
+)
+
+
+class QdeqError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"qdeq status {status}: {msg}")
+        self.status = status
+
+
+class QdeqConfig(ctypes.Structure):
+    _fields_ = [
+        ("ni", ctypes.c_int), ("nk", ctypes.c_int), ("dh", ctypes.c_double),
+        ("n_levels", ctypes.c_int), ("n_pre", ctypes.c_int),
+        ("n_post", ctypes.c_int), ("n_coarse", ctypes.c_int),
+        ("relax", ctypes.c_double), ("cycle", ctypes.c_int),
+        ("precision", ctypes.c_int), ("tol_mode", ctypes.c_int),
+        ("tol", ctypes.c_double), ("max_vcycles", ctypes.c_int),
+        ("ini_dt", ctypes.c_double), ("mindt", ctypes.c_double),
+        ("maxdt", ctypes.c_double), ("totTol", ctypes.c_double),
+        ("tmax", ctypes.c_double), ("f0", ctypes.c_double),
+        ("V0", ctypes.c_double), ("L", ctypes.c_double),
+        ("Vini", ctypes.c_double), ("device", ctypes.c_int),
+    ]
+
+
+class StepInfo(ctypes.Structure):
+    _fields_ = [
+        ("t", ctypes.c_double), ("dt", ctypes.c_double),
+        ("totErr", ctypes.c_double), ("errA0", ctypes.c_double),
+        ("errA1", ctypes.c_double), ("V_max", ctypes.c_double),
+        ("Ts_max", ctypes.c_double), ("it", ctypes.c_int),
+        ("attempts", ctypes.c_int), ("vcycles", ctypes.c_longlong),
+        ("solves", ctypes.c_longlong), ("ir_sweeps", ctypes.c_longlong),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Loads libqdeq.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} is missing: build it with qdeqcycle_fdm_amd.build.build()")
+    L = ctypes.CDLL(path)
+    L.qdeq_default_config.argtypes = [ctypes.POINTER(QdeqConfig)]
+    L.qdeq_default_config.restype = None
+    L.qdeq_create.argtypes = [ctypes.POINTER(QdeqConfig), _dp, _dp,
+                              ctypes.POINTER(_vp)]
+    L.qdeq_destroy.argtypes = [_vp]
+    L.qdeq_last_error.argtypes = [_vp]
+    L.qdeq_last_error.restype = ctypes.c_char_p
+    L.qdeq_global_error.argtypes = []
+    L.qdeq_global_error.restype = ctypes.c_char_p
+    L.qdeq_get_mu_equi.argtypes = [_vp, _vp, ctypes.c_int]
+    L.qdeq_build_mu_equi.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp,
+                                     ctypes.c_int]
+    L.qdeq_ode_solve.argtypes = [_vp, _vp, ctypes.c_int, _ip, _dp, _dp]
+    L.qdeq_set_boundary.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_double,
+                                    ctypes.c_double, ctypes.c_double, ctypes.c_int]
+    L.qdeq_eval_stress.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int]
+    L.qdeq_nr_search.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int]
+    L.qdeq_cal_dpsi.argtypes = [_vp, _vp, _vp, _vp, _vp, ctypes.c_int]
+    L.qdeq_compute_error.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
+    L.qdeq_compute_error.restype = ctypes.c_double
+    L.qdeq_compute_step_size.argtypes = [ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_double, _dp]
+    L.qdeq_compute_step_size.restype = ctypes.c_double
+    L.qdeq_rk43_setup.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp]
+    L.qdeq_rk43_step.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(StepInfo)]
+    L.qdeq_rk43_get_state.argtypes = [_vp, ctypes.POINTER(StepInfo), _dp, _dp,
+                                      _dp, _dp, _dp, _dp]
+    for name in EXPORTS:
+        getattr(L, name)  # every declared symbol must resolve
+    _lib = L
+    return L
+
+
+def _c(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def _vptr(a):
+    """Pointer of a numpy array (host) or a torch tensor (device)."""
+    if isinstance(a, np.ndarray):
+        assert a.dtype == np.float64 and a.flags.c_contiguous
+        return ctypes.c_void_p(a.ctypes.data), QDEQ_HOST
+    # torch tensor (device memory when a.is_cuda)
+    if str(a.dtype) != "torch.float64" or not a.is_contiguous():
+        raise TypeError("expected a contiguous float64 array or tensor")
+    return ctypes.c_void_p(a.data_ptr()), QDEQ_DEVICE if a.is_cuda else QDEQ_HOST
+
+
+def make_config(params: cfgmod.SolverParams, device: int = 0) -> QdeqConfig:
+    c = QdeqConfig()
+    load().qdeq_default_config(ctypes.byref(c))
+    c.ni, c.nk, c.dh = params.ni, params.nk, params.dh
+    c.n_levels = params.n_levels
+    c.n_pre, c.n_post, c.n_coarse = params.n_pre, params.n_post, params.n_coarse
+    c.relax = params.relax
+    c.cycle = params.cycle
+    c.precision = params.precision
+    c.tol_mode, c.tol = params.tol_mode, params.tol
+    c.max_vcycles = params.max_vcycles
+    c.ini_dt, c.mindt, c.maxdt = params.ini_dt, params.mindt, params.maxdt
+    c.totTol, c.tmax = params.totTol, params.tmax
+    c.f0, c.V0, c.L, c.Vini = params.f0, params.V0, params.L, params.Vini
+    c.device = device
+    return c
+
+
+def build_mu_equi(mu: np.ndarray, device: int = 0) -> np.ndarray:
+    """SetMedium's equivalent-medium array, built on the GPU."""
+    mu = _c(mu)
+    nz, nx = mu.shape
+    out = np.zeros((2 * nz - 1, 2 * nx - 1))
+    st = load().qdeq_build_mu_equi(nx - 2, nz - 2, _p(mu), _p(out), device)
+    if st != QDEQ_OK:
+        raise QdeqError(st, load().qdeq_global_error().decode())
+    return out
+
+
+class Engine:
+    """One simulation context on one GPU.
+
+    ``problem`` supplies the grid, medium and friction arrays
+    (:mod:`qdeqcycle_fdm_amd.config`); ``mu_equi`` is optional (built on the
+    device when omitted).
+    """
+
+    def __init__(self, problem: cfgmod.Problem, mu_equi: Optional[np.ndarray] = None,
+                 *, device: int = 0, **overrides):
+        import dataclasses
+        params = dataclasses.replace(problem.params, **overrides)
+        self.problem = problem
+        self.params = params
+        self.cfg = make_config(params, device)
+        self.nx, self.nz = params.nx, params.nz
+        self._mu = _c(problem.mu)
+        me = None if mu_equi is None else _c(mu_equi)
+        h = _vp()
+        L = load()
+        st = L.qdeq_create(ctypes.byref(self.cfg), _p(self._mu),
+                           None if me is None else _p(me), ctypes.byref(h))
+        if st != QDEQ_OK:
+            raise QdeqError(st, L.qdeq_global_error().decode())
+        self.h = h
+
+    # -- errors --
+    def _check(self, st: int):
+        if st != QDEQ_OK:
+            raise QdeqError(st, load().qdeq_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().qdeq_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def mu_equi(self) -> np.ndarray:
+        out = np.zeros((2 * self.nz - 1, 2 * self.nx - 1))
+        self._check(load().qdeq_get_mu_equi(self.h, ctypes.c_void_p(out.ctypes.data),
+                                            QDEQ_HOST))
+        return out
+
+    # -- drop-in functions (reference names) --
+    def ode_solve(self, U):
+        """ode_solve (odeSolver.cpp:605): in place on U (nz, nx); returns
+        (vcycles, res0, res)."""
+        ptr, where = _vptr(U)
+        vc = ctypes.c_int(0)
+        r0 = ctypes.c_double(0)
+        r = ctypes.c_double(0)
+        self._check(load().qdeq_ode_solve(self.h, ptr, where, ctypes.byref(vc),
+                                          ctypes.byref(r0), ctypes.byref(r)))
+        return vc.value, r0.value, r.value
+
+    def setBoundary(self, U, slip, slip_shift, t, ratio, dt):
+        pu, where = _vptr(U)
+        ps, _ = _vptr(slip)
+        pss, _ = _vptr(slip_shift)
+        self._check(load().qdeq_set_boundary(self.h, pu, ps, pss, t, ratio, dt, where))
+
+    def eval_stress_on_fault(self, U, mu, tau_n0):
+        U, mu, tau_n0 = _c(U), _c(mu), _c(tau_n0)
+        Ts = np.zeros(self.nz)
+        Tn = np.zeros(self.nz)
+        args = [ctypes.c_void_p(a.ctypes.data) for a in (U, mu, tau_n0, Ts, Tn)]
+        self._check(load().qdeq_eval_stress(self.h, *args, QDEQ_HOST))
+        return Ts, Tn
+
+    def NRsearch(self, Ts, Tn, psi, a, eta, V=None):
+        V = np.zeros(self.nz) if V is None else _c(V).copy()
+        arrs = [_c(x) for x in (Ts, Tn, psi, a, eta)]
+        args = [ctypes.c_void_p(x.ctypes.data) for x in arrs]
+        self._check(load().qdeq_nr_search(self.h, *args, ctypes.c_void_p(V.ctypes.data),
+                                          QDEQ_HOST))
+        return V
+
+    def cal_dpsi(self, psi, V, b):
+        arrs = [_c(x) for x in (psi, V, b)]
+        d = np.zeros(self.nz)
+        args = [ctypes.c_void_p(x.ctypes.data) for x in arrs]
+        self._check(load().qdeq_cal_dpsi(self.h, *args, ctypes.c_void_p(d.ctypes.data),
+                                         QDEQ_HOST))
+        return d
+
+    @staticmethod
+    def computeError(s3, p3, s4, p4) -> float:
+        s3, p3, s4, p4 = (_c(x) for x in (s3, p3, s4, p4))
+        return load().qdeq_compute_error(len(s3) - 2, _p(s3), _p(p3), _p(s4), _p(p4))
+
+    @staticmethod
+    def computeStepSize(it, dt, mindt, maxdt, totErr, totTol, errA) -> float:
+        e = _c(errA)
+        return load().qdeq_compute_step_size(it, dt, mindt, maxdt, totErr, totTol, _p(e))
+
+    # -- RK43 driver --
+    def rk43_setup(self):
+        pb = self.problem
+        self._fault = [_c(pb.a), _c(pb.b), _c(pb.eta), _c(pb.tau_n0), _c(pb.U0)]
+        self._check(load().qdeq_rk43_setup(self.h, *[_p(x) for x in self._fault]))
+
+    def rk43_step(self, nsteps: int = 1) -> dict:
+        info = StepInfo()
+        self._check(load().qdeq_rk43_step(self.h, nsteps, ctypes.byref(info)))
+        return info.as_dict()
+
+    def state(self) -> dict:
+        info = StepInfo()
+        U = np.zeros((self.nz, self.nx))
+        out = {k: np.zeros(self.nz) for k in ("slip", "psi", "V", "dpsi", "Ts")}
+        self._check(load().qdeq_rk43_get_state(
+            self.h, ctypes.byref(info), _p(U), _p(out["slip"]), _p(out["psi"]),
+            _p(out["V"]), _p(out["dpsi"]), _p(out["Ts"])))
+        out["U"] = U
+        out["info"] = info.as_dict()
+        return out
