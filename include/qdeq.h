@@ -95,6 +95,15 @@ void qdeq_default_config(qdeq_config *cfg);
 int qdeq_create(const qdeq_config *cfg, const double *mu,
                 const double *mu_equi, qdeq_ctx **out);
 int qdeq_destroy(qdeq_ctx *ctx);
+/* Execution shape of the fast cycle (no reference counterpart): tiled != 0
+ * runs it on the temporally blocked LDS kernels with interior tiles of
+ * tile_x by tile_y points (even sizes, limited by the 160 KiB LDS at the
+ * configured sweep counts); tiled == 0 uses one launch per half-sweep.
+ * Results are identical either way. */
+int qdeq_set_tiles(qdeq_ctx *ctx, int tiled, int tile_x, int tile_y);
+/* 1 if solves of ctx currently run the fast cycle on the tiled kernels. */
+int qdeq_uses_tiled_cycle(const qdeq_ctx *ctx);
+
 /* Message of the last failing call on ctx ("" if none). */
 const char *qdeq_last_error(const qdeq_ctx *ctx);
 /* Message of the last failing qdeq_create / context-free call. */

@@ -32,7 +32,9 @@ class OrcParams(ctypes.Structure):
         ("maxdt", ctypes.c_double), ("totTol", ctypes.c_double),
         ("tmax", ctypes.c_double), ("f0", ctypes.c_double),
         ("V0", ctypes.c_double), ("L", ctypes.c_double),
-        ("Vini", ctypes.c_double),
+        ("Vini", ctypes.c_double), ("cycle", ctypes.c_int),
+        ("n_pre", ctypes.c_int), ("n_post", ctypes.c_int),
+        ("n_coarse", ctypes.c_int),
     ]
 
 
@@ -72,6 +74,11 @@ def lib():
                                     _dp]
         L.orc_mg_create.restype = ctypes.c_void_p
         L.orc_mg_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_mg_set_cycle.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int]
+        L.orc_mg_set_cycle.restype = None
+        L.orc_prolong_full.argtypes = [_dp, _dp, ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int]
         L.orc_mg_vcycle.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.orc_ode_solve.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int,
                                     ctypes.c_double, ctypes.c_int, _ip, _dp, _dp]
@@ -231,6 +238,10 @@ class MG:
         if not self.h:
             raise ValueError("orc_mg_create failed")
 
+    def set_cycle(self, n_pre, n_post, n_coarse, full_top=True):
+        """The engine's fast cycle shape (no reference counterpart)."""
+        lib().orc_mg_set_cycle(self.h, n_pre, n_post, n_coarse, 1 if full_top else 0)
+
     def vcycle(self, u, f):
         lib().orc_mg_vcycle(self.h, _p(u), _p(_c(f)))
 
@@ -262,7 +273,8 @@ class Sim:
     """Oracle RK43 simulation (odeSolver.cpp:1277-1674)."""
 
     def __init__(self, problem, mu_equi_arr=None, *, tol_mode=None, tol=None,
-                 max_vcycles=None, n_cycles=None):
+                 max_vcycles=None, n_cycles=None, cycle=None, n_pre=None,
+                 n_post=None, n_coarse=None):
         p = problem.params
         self.problem = problem
         me = mu_equi(problem.mu) if mu_equi_arr is None else mu_equi_arr
@@ -276,7 +288,11 @@ class Sim:
             tol=p.tol if tol is None else tol,
             max_vcycles=p.max_vcycles if max_vcycles is None else max_vcycles,
             ini_dt=p.ini_dt, mindt=p.mindt, maxdt=p.maxdt, totTol=p.totTol,
-            tmax=p.tmax, f0=p.f0, V0=p.V0, L=p.L, Vini=p.Vini)
+            tmax=p.tmax, f0=p.f0, V0=p.V0, L=p.L, Vini=p.Vini,
+            cycle=p.cycle if cycle is None else cycle,
+            n_pre=p.n_pre if n_pre is None else n_pre,
+            n_post=p.n_post if n_post is None else n_post,
+            n_coarse=p.n_coarse if n_coarse is None else n_coarse)
         self.h = lib().orc_sim_create(ctypes.byref(self.params),
                                       *[_p(a) for a in self._arrs])
         self.nx, self.nz = p.nx, p.nz

@@ -360,8 +360,11 @@ void orc_restrict(double *r2, const double *r, double *e2, int ni, int nk,
 
 /* prolongate_kernel, odeSolver.cpp:1017-1043 (level = coarse level).
  * Bilinear correction over coarse j < Nc-1, k < Kc-1: the last fine row and
- * column receive no correction. */
-void orc_prolong(double *u, const double *e2, int ni, int nk, int level) {
+ * column receive no correction.  With full_top (the engine's fast cycle,
+ * no reference counterpart) the last coarse row also corrects the last fine
+ * row. */
+void orc_prolong_full(double *u, const double *e2, int ni, int nk, int level,
+                      int full_top) {
   int N[ORC_MAX_LEVELS], K[ORC_MAX_LEVELS], FC[ORC_MAX_LEVELS];
   double DH[ORC_MAX_LEVELS];
   orc_level_dims(ni, nk, 1.0, level + 1, N, K, FC, DH);
@@ -376,8 +379,19 @@ void orc_prolong(double *u, const double *e2, int ni, int nk, int level) {
       UF(2 * j + 1, 2 * k + 1) +=
           0.25 * (EC(j, k) + EC(j, k + 1) + EC(j + 1, k) + EC(j + 1, k + 1));
     }
+  if (full_top) {
+    const long j = Nc - 1;
+    for (long k = 0; k < Kc - 1; k++) {
+      UF(2 * j, 2 * k) += EC(j, k);
+      UF(2 * j, 2 * k + 1) += 0.5 * (EC(j, k) + EC(j, k + 1));
+    }
+  }
 #undef UF
 #undef EC
+}
+
+void orc_prolong(double *u, const double *e2, int ni, int nk, int level) {
+  orc_prolong_full(u, e2, ni, nk, level, 0);
 }
 
 /* Euclidean norm standing in for cublasDnrm2 (odeSolver.cpp:694,700); the
@@ -391,6 +405,7 @@ double orc_nrm2(const double *x, long n) {
 
 struct orc_mg {
   int ni, nk, L, nu;
+  int n_pre, n_post, n_coarse, full_top;  /* cycle shape (reference: nu,nu,nu,0) */
   double dh, relax;
   int N[ORC_MAX_LEVELS], K[ORC_MAX_LEVELS], FC[ORC_MAX_LEVELS];
   double DH[ORC_MAX_LEVELS];
@@ -406,6 +421,8 @@ orc_mg *orc_mg_create(int ni, int nk, double dh, int n_levels, int n_cycles,
   if (n_levels < 1 || n_levels > ORC_MAX_LEVELS) return NULL;
   orc_mg *mg = (orc_mg *)calloc(1, sizeof(orc_mg));
   mg->ni = ni; mg->nk = nk; mg->L = n_levels; mg->nu = n_cycles;
+  mg->n_pre = mg->n_post = mg->n_coarse = n_cycles;
+  mg->full_top = 0;
   mg->dh = dh; mg->relax = relax;
   orc_level_dims(ni, nk, dh, n_levels, mg->N, mg->K, mg->FC, mg->DH);
   const size_t nmu = (size_t)(2 * (ni + 2) - 1) * (size_t)(2 * (nk + 2) - 1);
@@ -434,9 +451,18 @@ void orc_mg_destroy(orc_mg *mg) {
   free(mg);
 }
 
-/* smooth, odeSolver.cpp:741-755: n_cycles x (red, black). */
-static void orc_smooth(orc_mg *mg, double *u, const double *f, int level) {
-  for (int it = 0; it < mg->nu; it++) {
+void orc_mg_set_cycle(orc_mg *mg, int n_pre, int n_post, int n_coarse,
+                      int full_top) {
+  mg->n_pre = n_pre;
+  mg->n_post = n_post;
+  mg->n_coarse = n_coarse;
+  mg->full_top = full_top;
+}
+
+/* smooth, odeSolver.cpp:741-755: sweeps x (red, black). */
+static void orc_smooth(orc_mg *mg, double *u, const double *f, int level,
+                       int sweeps) {
+  for (int it = 0; it < sweeps; it++) {
     orc_rbgs(u, f, mg->mu, mg->ni, mg->nk, mg->dh, level, 0, mg->relax);
     orc_rbgs(u, f, mg->mu, mg->ni, mg->nk, mg->dh, level, 1, mg->relax);
   }
@@ -446,16 +472,17 @@ static void orc_smooth(orc_mg *mg, double *u, const double *f, int level) {
 static void orc_vcycle_level(orc_mg *mg, double *u, const double *f,
                              int level) {
   if (level >= mg->L - 1) {
-    orc_smooth(mg, u, f, level);
+    orc_smooth(mg, u, f, level, mg->n_coarse);
     return;
   }
-  orc_smooth(mg, u, f, level);
+  orc_smooth(mg, u, f, level, mg->n_pre);
   orc_residual(mg->res[level], u, f, mg->mu, mg->ni, mg->nk, mg->dh, level);
   orc_restrict(mg->res2[level], mg->res[level], mg->err2[level], mg->ni,
                mg->nk, level + 1);
   orc_vcycle_level(mg, mg->err2[level], mg->res2[level], level + 1);
-  orc_prolong(u, mg->err2[level], mg->ni, mg->nk, level + 1);
-  orc_smooth(mg, u, f, level);
+  orc_prolong_full(u, mg->err2[level], mg->ni, mg->nk, level + 1,
+                   mg->full_top);
+  orc_smooth(mg, u, f, level, mg->n_post);
 }
 
 void orc_mg_vcycle(orc_mg *mg, double *u, const double *f) {
@@ -915,6 +942,8 @@ orc_sim *orc_sim_create(const orc_params *p, const double *mu,
   s->t = 0.0;
   s->mg = orc_mg_create(p->ni, p->nk, p->dh, p->n_levels, p->n_cycles,
                         p->relax, mu_equi);
+  if (p->cycle == 1)
+    orc_mg_set_cycle(s->mg, p->n_pre, p->n_post, p->n_coarse, 1);
   return s;
 }
 

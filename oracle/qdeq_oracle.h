@@ -50,6 +50,10 @@ typedef struct orc_params {
   int max_vcycles;     /* V-cycle cap per solve (the reference has none) */
   double ini_dt, mindt, maxdt, totTol, tmax;   /* SetParams.cpp:5-18 */
   double f0, V0, L, Vini;                      /* fault.cpp:85-89 */
+  /* cycle shape: 0 = the reference's (n_cycles sweeps everywhere, last fine
+   * row uncorrected); 1 = the engine's fast cycle (n_pre / n_post /
+   * n_coarse sweeps, every fine row corrected) */
+  int cycle, n_pre, n_post, n_coarse;
 } orc_params;
 
 /* ---- medium (odeSolver.cpp:337-589) ---- */
@@ -65,6 +69,8 @@ void orc_residual(double *r, const double *u, const double *f,
 void orc_restrict(double *r2, const double *r, double *e2, int ni, int nk,
                   int level);
 void orc_prolong(double *u, const double *e2, int ni, int nk, int level);
+void orc_prolong_full(double *u, const double *e2, int ni, int nk, int level,
+                      int full_top);
 double orc_nrm2(const double *x, long n);
 
 /* opaque multigrid solver: owns the level buffers (odeSolver.cpp:6-8,1391-1405) */
@@ -72,6 +78,9 @@ typedef struct orc_mg orc_mg;
 orc_mg *orc_mg_create(int ni, int nk, double dh, int n_levels, int n_cycles,
                       double relax, const double *mu_equi);
 void orc_mg_destroy(orc_mg *mg);
+/* sweep counts and top-row correction of the cycle (default: the reference's) */
+void orc_mg_set_cycle(orc_mg *mg, int n_pre, int n_post, int n_coarse,
+                      int full_top);
 /* one V-cycle on the compact level-0 arrays (odeSolver.cpp:1101-1147) */
 void orc_mg_vcycle(orc_mg *mg, double *u, const double *f);
 /* ode_solve = multigrid + extend_crew on the nx*nz array U

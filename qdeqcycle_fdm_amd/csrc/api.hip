@@ -116,6 +116,8 @@ int qdeq_create(const qdeq_config *cfg, const double *mu,
   c->nx = cfg->ni + 2;
   c->nz = cfg->nk + 2;
   c->L = cfg->n_levels;
+  c->tile_tx = 64;
+  c->tile_ty = cfg->precision == QDEQ_PREC_FP32_IR ? 32 : 16;
   {
     int n = cfg->nk, k = cfg->ni, fc = 1;
     double h = cfg->dh;
@@ -168,7 +170,7 @@ int qdeq_create(const qdeq_config *cfg, const double *mu,
       return QDEQ_ERR_HIP;
     }
   }
-  CREATE_HIP(alloc_zero(&c->partial, 2 * 1024, c->stream));
+  CREATE_HIP(alloc_zero(&c->partial, 16384, c->stream));
   CREATE_HIP(alloc_zero(&c->scal, SC_COUNT, c->stream));
   CREATE_HIP(alloc_zero(&c->flags, 4, c->stream));
   CREATE_HIP(hipHostMalloc(&c->scal_host, SC_COUNT * sizeof(double)));
@@ -209,7 +211,7 @@ int qdeq_create(const qdeq_config *cfg, const double *mu,
                               hipMemcpyHostToDevice, c->stream));
     CREATE_HIP(hipStreamSynchronize(c->stream));
   }
-  if (mg_build_coefficients(c) != QDEQ_OK) {
+  if (mg_build_coefficients(c) != QDEQ_OK || tiled_setup(c) != QDEQ_OK) {
     set_error(nullptr, QDEQ_ERR_HIP, c->err);
     free_ctx(c);
     return QDEQ_ERR_HIP;
@@ -222,6 +224,22 @@ int qdeq_destroy(qdeq_ctx *ctx) {
   free_ctx(ctx);
   return QDEQ_OK;
 }
+
+int qdeq_set_tiles(qdeq_ctx *c, int tiled, int tile_x, int tile_y) {
+  if (!c) return set_error(c, QDEQ_ERR_ARG, "null argument");
+  if (tile_x < 8 || tile_y < 2 || (tile_x & 1) || (tile_y & 1) || tile_x > 256 ||
+      tile_y > 128)
+    return set_error(c, QDEQ_ERR_ARG, "tile sizes must be even, 8..256 x 2..128");
+  if (!tiles_fit(c, tile_x, tile_y))
+    return set_error(c, QDEQ_ERR_ARG, "tile too large for the LDS at these sweep counts");
+  c->tiled = tiled != 0;
+  c->tile_tx = tile_x;
+  c->tile_ty = tile_y;
+  mg_destroy_graphs(c);
+  return QDEQ_OK;
+}
+
+int qdeq_uses_tiled_cycle(const qdeq_ctx *c) { return c && use_tiled(c) ? 1 : 0; }
 
 int qdeq_get_mu_equi(qdeq_ctx *c, double *me, int where) {
   if (!c || !me) return set_error(c, QDEQ_ERR_ARG, "null argument");

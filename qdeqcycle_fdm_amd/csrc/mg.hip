@@ -345,19 +345,34 @@ static void launch_bdres(qdeq_ctx *c) {
                      c->partial + NORM_BLOCKS, nb, c->scal + SC_NORM0);
 }
 
+// Whether this context runs the fast cycle with the tiled kernels.
+bool use_tiled(const qdeq_ctx *c) {
+  return c->cfg.cycle == QDEQ_CYCLE_FAST && c->tiled &&
+         tail_fits(c, c->cfg.precision != QDEQ_PREC_FP32_IR);
+}
+
 // The launch sequence of one solver iteration, followed by the residual norm.
+// FP64 cycles always use the reference's update formula (bit-comparable to
+// the oracle); the FP32 correction cycle uses a plain division.
 static void launch_iteration(qdeq_ctx *c) {
+  const bool tiled = use_tiled(c);
   if (c->cfg.precision == QDEQ_PREC_FP32_IR) {
-    vcycle<float, false>(c, c->s, 0, true);
-    const LevelDims &d = c->lv[0];
-    hipLaunchKernelGGL(k_add_correction, grid_cols(d.K - 2, d.N), dim3(BX, BY),
-                       0, c->stream, c->d[0].u, c->s[0].u, d.N, d.K, d.P);
+    if (tiled) {
+      vcycle_tiled_fp32_ir(c);
+    } else {
+      vcycle<float, false>(c, c->s, 0, true);
+      const LevelDims &d = c->lv[0];
+      hipLaunchKernelGGL(k_add_correction, grid_cols(d.K - 2, d.N), dim3(BX, BY),
+                         0, c->stream, c->d[0].u, c->s[0].u, d.N, d.K, d.P);
+    }
     launch_resnorm(c, true);
+  } else if (tiled) {
+    int nparts = 0;
+    vcycle_tiled_fp64(c, c->partial, &nparts);
+    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(NORM_THREADS), 0, c->stream,
+                       c->partial, nparts, c->scal + SC_NORM);
   } else {
-    if (c->cfg.cycle == QDEQ_CYCLE_REFERENCE)
-      vcycle<double, true>(c, c->d, 0, false);
-    else
-      vcycle<double, false>(c, c->d, 0, false);
+    vcycle<double, true>(c, c->d, 0, false);
     launch_resnorm(c, false);
   }
 }
@@ -369,7 +384,8 @@ void mg_destroy_graphs(qdeq_ctx *c) {
 }
 
 static int ensure_graph(qdeq_ctx *c) {
-  const int mode = c->cfg.precision * 2 + c->cfg.cycle;
+  const int mode = c->cfg.precision * 2 + c->cfg.cycle + (c->tiled ? 4 : 0) +
+                   8 * (c->tile_tx * 1024 + c->tile_ty);
   if (c->g_cycle && c->g_cycle_mode == mode) return QDEQ_OK;
   mg_destroy_graphs(c);
   hipGraph_t g = nullptr;

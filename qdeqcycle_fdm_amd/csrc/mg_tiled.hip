@@ -1,0 +1,600 @@
+// mg_tiled.hip -- the fast V-cycle: temporally blocked LDS tile kernels.
+//
+// One V-cycle of the fast cycle costs two launches per level above the coarse
+// tail plus one launch for the whole tail:
+//   k_down  (level l): nu red/black sweeps, residual, full-weighting
+//           restriction to level l+1, all on one LDS tile with a halo of
+//           2 nu + 2 points, so the ring of redundant updates makes every
+//           interior value exactly what global sweeps would produce;
+//   k_tail  (levels with N <= TAIL_MAX_N): one workgroup runs the rest of the
+//           V-cycle in LDS with workgroup barriers only;
+//   k_up    (level l): bilinear prolongation + correction, nu sweeps, and on
+//           level 0 either the residual-norm partials (FP64 solve) or the
+//           FP64 update u += e (FP32 correction of iterative refinement).
+// Every update, residual, restriction and prolongation is evaluated with the
+// same expressions as the one-launch-per-half-sweep kernels of mg.hip, so the
+// tiled cycle reproduces them bit for bit.  The input and output u of a tile
+// kernel are different buffers (tiles read their neighbours' interiors as
+// halo).
+#include <cmath>
+
+#include "qdeq_internal.h"
+
+namespace qdeq {
+
+constexpr int TILE_THREADS = 256;
+constexpr int TAIL_THREADS = 1024;
+
+template <typename T, bool EXACT>
+__device__ inline T rbgs_update(T m1, T m2, T m3, T m4, T uW, T uE, T uS,
+                                T uN, T uC, T fDH2, bool has_f, T relax) {
+  T num = m1 * uW + m2 * uE + m3 * uS + m4 * uN;
+  if (has_f) num = num - fDH2;
+  if (EXACT) return (relax / (m1 + m2 + m3 + m4)) * num + (T(1) - relax) * uC;
+  const T unew = num / (m1 + m2 + m3 + m4);
+  return (relax == T(1)) ? unew : uC + relax * (unew - uC);
+}
+
+// Shared layout of a tile: rows [0, R), columns [0, C), row stride LS.
+// su, sf, scx hold R rows; scz holds R+1 rows (south faces; row lr+1 is the
+// north face of row lr).  Global row of local row lr is j0 + lr.
+template <typename T>
+struct Tile {
+  T *su, *sf, *scx, *scz;
+  int R, C, LS, j0, k0;
+};
+
+template <typename T, bool HAS_F>
+__device__ inline void tile_load(Tile<T> &t, const T *__restrict__ u,
+                                 const T *__restrict__ f,
+                                 const T *__restrict__ cx,
+                                 const T *__restrict__ cz, int N, int K, int P,
+                                 bool zero_u) {
+  const int total = t.R * t.C;
+  for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+    const int lr = idx / t.C, lc = idx - lr * t.C;
+    const int j = t.j0 + lr, k = t.k0 + lc;
+    const int s = lr * t.LS + lc;
+    const bool in = (j >= 0) && (j < N) && (k >= 0) && (k < K);
+    const size_t g = (size_t)j * P + k;
+    t.su[s] = (in && !zero_u) ? u[g] : T(0);
+    if (HAS_F) t.sf[s] = in ? f[g] : T(0);
+    t.scx[s] = in ? cx[g] : T(0);
+    t.scz[s] = (j >= 0 && j <= N && k >= 0 && k < K) ? cz[g] : T(0);
+  }
+  for (int lc = threadIdx.x; lc < t.C; lc += blockDim.x) {
+    const int j = t.j0 + t.R, k = t.k0 + lc;
+    const bool in = (j >= 0) && (j <= N) && (k >= 0) && (k < K);
+    t.scz[t.R * t.LS + lc] = in ? cz[(size_t)j * P + k] : T(0);
+  }
+}
+
+// Half-sweep `colour` over the local window [h, R-h) x [h, C-h).
+template <typename T, bool HAS_F, bool EXACT>
+__device__ inline void tile_half_sweep(Tile<T> &t, int h, int colour, int N,
+                                       int K, T DH2, T relax) {
+  const int rows = t.R - 2 * h, cols = t.C - 2 * h;
+  if (rows <= 0 || cols <= 0) return;
+  const int half = (cols + 1) / 2;
+  const int total = rows * half;
+  for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+    const int rr = idx / half, q = idx - rr * half;
+    const int lr = h + rr;
+    const int j = t.j0 + lr;
+    const int lc = h + 2 * q + ((colour + j + t.k0 + h) & 1);
+    if (lc >= t.C - h) continue;
+    const int k = t.k0 + lc;
+    if (j < 0 || j >= N || k < 1 || k > K - 2) continue;
+    const int lS = (j == 0) ? lr + 1 : lr - 1;
+    const int lN = (j == N - 1) ? lr - 1 : lr + 1;
+    const int s = lr * t.LS + lc;
+    const T m1 = t.scx[s], m2 = t.scx[s + 1], m3 = t.scz[s], m4 = t.scz[s + t.LS];
+    const T fv = HAS_F ? DH2 * t.sf[s] : T(0);
+    t.su[s] = rbgs_update<T, EXACT>(m1, m2, m3, m4, t.su[s - 1], t.su[s + 1],
+                                    t.su[lS * t.LS + lc], t.su[lN * t.LS + lc],
+                                    t.su[s], fv, HAS_F, relax);
+  }
+}
+
+template <typename T, bool HAS_F, bool EXACT>
+__device__ inline void tile_sweeps(Tile<T> &t, int nu, int N, int K, T DH2,
+                                   T relax) {
+  for (int s = 0; s < 2 * nu; s++) {
+    tile_half_sweep<T, HAS_F, EXACT>(t, s + 1, s & 1, N, K, DH2, relax);
+    __syncthreads();
+  }
+}
+
+// residual of local point (lr, lc) (global row j), same expression as
+// k_residual.
+template <typename T, bool HAS_F>
+__device__ inline T tile_residual(const Tile<T> &t, int lr, int lc, int j,
+                                  int N, T DH2) {
+  const int lS = (j == 0) ? lr + 1 : lr - 1;
+  const int lN = (j == N - 1) ? lr - 1 : lr + 1;
+  const int s = lr * t.LS + lc;
+  const T m1 = t.scx[s], m2 = t.scx[s + 1], m3 = t.scz[s], m4 = t.scz[s + t.LS];
+  const T fv = HAS_F ? t.sf[s] : T(0);
+  return fv - (m1 * t.su[s - 1] + m2 * t.su[s + 1] + m3 * t.su[lS * t.LS + lc] +
+               m4 * t.su[lN * t.LS + lc] - (m1 + m2 + m3 + m4) * t.su[s]) /
+                  DH2;
+}
+
+template <typename T>
+__device__ inline Tile<T> make_tile(unsigned char *smem, int TX, int TY, int H,
+                                    bool has_f) {
+  Tile<T> t;
+  t.R = TY + 2 * H;
+  t.C = TX + 2 * H;
+  t.LS = t.C;
+  t.j0 = (int)blockIdx.y * TY - H;
+  t.k0 = (int)blockIdx.x * TX - H;
+  T *p = reinterpret_cast<T *>(smem);
+  const int n = t.R * t.LS;
+  t.su = p;
+  t.scx = p + n;
+  t.scz = p + 2 * n;
+  t.sf = has_f ? p + 3 * n + t.LS : nullptr;
+  return t;
+}
+
+// Pre-smoothing + residual + restriction (levels above the tail).
+template <typename T, bool HAS_F, bool EXACT>
+__global__ __launch_bounds__(TILE_THREADS) void k_down(
+    const T *__restrict__ uin, T *__restrict__ uout, const T *__restrict__ f,
+    const T *__restrict__ cx, const T *__restrict__ cz, int N, int K, int P,
+    T DH2, T relax, int nu, int zero_u, T *__restrict__ r2, int Nc, int Kc,
+    int Pc, int TX, int TY) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int H = 2 * nu + 2;
+  Tile<T> t = make_tile<T>(smem, TX, TY, H, HAS_F);
+  tile_load<T, HAS_F>(t, uin, f, cx, cz, N, K, P, zero_u != 0);
+  __syncthreads();
+  tile_sweeps<T, HAS_F, EXACT>(t, nu, N, K, DH2, relax);
+  // residual on the interior plus one ring, kept in sf's place when HAS_F,
+  // or in the scratch row block after scz otherwise
+  T *sr = HAS_F ? t.sf : t.scz + (t.R + 1) * t.LS;
+  const int h = H - 1;
+  {
+    const int rows = t.R - 2 * h, cols = t.C - 2 * h;
+    for (int idx = threadIdx.x; idx < rows * cols; idx += blockDim.x) {
+      const int lr = h + idx / cols, lc = h + idx % cols;
+      const int j = t.j0 + lr, k = t.k0 + lc;
+      T r = T(0);
+      if (j >= 0 && j < N && k >= 1 && k <= K - 2)
+        r = tile_residual<T, HAS_F>(t, lr, lc, j, N, DH2);
+      // the residual overwrites f only after every thread has read f at this
+      // point: f is read at the point itself alone
+      sr[lr * t.LS + lc] = r;
+    }
+  }
+  __syncthreads();
+  // restriction for the coarse points inside the tile interior
+  const int ncr = TY / 2, ncc = TX / 2;
+  const int jc0 = (t.j0 + H) / 2, kc0 = (t.k0 + H) / 2;
+  for (int idx = threadIdx.x; idx < ncr * ncc; idx += blockDim.x) {
+    const int jc = jc0 + idx / ncc, kc = kc0 + idx % ncc;
+    if (jc >= Nc || kc < 1 || kc > Kc - 2) continue;
+    const int jS = (jc == 0) ? 1 : 2 * jc - 1;
+    const int jN = (jc == Nc - 1) ? 2 * jc - 1 : 2 * jc + 1;
+    const T *rc = sr + (2 * jc - t.j0) * t.LS - t.k0;
+    const T *rs = sr + (jS - t.j0) * t.LS - t.k0;
+    const T *rn = sr + (jN - t.j0) * t.LS - t.k0;
+    const int kf = 2 * kc;
+    const T v = T(0.0625) *
+                (T(4.0) * rc[kf] + T(2.0) * rc[kf - 1] + T(2.0) * rc[kf + 1] +
+                 T(2.0) * rs[kf] + T(2.0) * rn[kf] + rs[kf - 1] + rs[kf + 1] +
+                 rn[kf - 1] + rn[kf + 1]);
+    r2[(size_t)jc * Pc + kc] = v;
+  }
+  // smoothed interior
+  for (int idx = threadIdx.x; idx < TY * TX; idx += blockDim.x) {
+    const int lr = H + idx / TX, lc = H + idx % TX;
+    const int j = t.j0 + lr, k = t.k0 + lc;
+    if (j < N && k < K) uout[(size_t)j * P + k] = t.su[lr * t.LS + lc];
+  }
+}
+
+// Level-0 finishing modes of k_up.
+enum { UP_STORE = 0, UP_NORM = 1, UP_ADD64 = 2 };
+
+__device__ inline double block_sum(double v, double *sh) {
+  const int tid = threadIdx.x;
+  sh[tid] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (tid < s) sh[tid] += sh[tid + s];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+// Prolongation + correction + post-smoothing.
+template <typename T, bool HAS_F, bool EXACT, bool FULL_TOP, int MODE>
+__global__ __launch_bounds__(TILE_THREADS) void k_up(
+    const T *__restrict__ uin, T *__restrict__ uout, const T *__restrict__ f,
+    const T *__restrict__ cx, const T *__restrict__ cz, int N, int K, int P,
+    T DH2, T relax, int nu, const T *__restrict__ e, int Nc, int Kc, int Pc,
+    int TX, int TY, double *__restrict__ partial, double *__restrict__ u64) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double red[TILE_THREADS];
+  const int H = 2 * nu + (MODE == UP_NORM ? 1 : 0);
+  Tile<T> t = make_tile<T>(smem, TX, TY, H, HAS_F);
+  tile_load<T, HAS_F>(t, uin, f, cx, cz, N, K, P, false);
+  __syncthreads();
+  // u += P e (prolongate_kernel, odeSolver.cpp:1017-1043), point by point
+  for (int idx = threadIdx.x; idx < t.R * t.C; idx += blockDim.x) {
+    const int lr = idx / t.C, lc = idx - lr * t.C;
+    const int j = t.j0 + lr, k = t.k0 + lc;
+    if (j < 0 || j >= N || k < 0 || k >= K) continue;
+    const int cj = j >> 1, ck = k >> 1, dj = j & 1, dk = k & 1;
+    if (ck >= Kc - 1) continue;
+    if (!(cj < Nc - 1 || (FULL_TOP && cj == Nc - 1 && dj == 0))) continue;
+    const T *ec = e + (size_t)cj * Pc + ck;
+    const T e00 = ec[0];
+    T add;
+    if (!dj && !dk) {
+      add = e00;
+    } else if (!dj) {
+      add = T(0.5) * (e00 + ec[1]);
+    } else if (!dk) {
+      add = T(0.5) * (e00 + ec[Pc]);
+    } else {
+      add = T(0.25) * (e00 + ec[1] + ec[Pc] + ec[Pc + 1]);
+    }
+    const int s = lr * t.LS + lc;
+    t.su[s] += add;
+  }
+  __syncthreads();
+  tile_sweeps<T, HAS_F, EXACT>(t, nu, N, K, DH2, relax);
+  double acc = 0.0;
+  for (int idx = threadIdx.x; idx < TY * TX; idx += blockDim.x) {
+    const int lr = H + idx / TX, lc = H + idx % TX;
+    const int j = t.j0 + lr, k = t.k0 + lc;
+    if (j >= N || k >= K) continue;
+    const size_t g = (size_t)j * P + k;
+    if (MODE == UP_NORM && k >= 1 && k <= K - 2) {
+      const double r = (double)tile_residual<T, HAS_F>(t, lr, lc, j, N, DH2);
+      acc += r * r;
+    }
+    if (MODE == UP_ADD64) {
+      if (k >= 1 && k <= K - 2) u64[g] += (double)t.su[lr * t.LS + lc];
+    } else {
+      uout[g] = t.su[lr * t.LS + lc];
+    }
+  }
+  if (MODE == UP_NORM) {
+    const double s = block_sum(acc, red);
+    if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Coarse tail: one workgroup, every level in LDS.
+// ---------------------------------------------------------------------------
+
+struct TailLevel {
+  const void *cx, *cz;  // global coefficients of the level
+  int N, K, P;          // dims and global pitch
+  double DH2;
+};
+
+struct TailArgs {
+  TailLevel lv[6];
+  int nlev;             // levels in the tail (first = entry level)
+  const void *f_in;     // restricted residual of the entry level (global)
+  void *e_out;          // correction of the entry level (global)
+  int nu_pre, nu_post, nu_coarse;
+  double relax;
+};
+
+template <typename T, bool EXACT>
+__device__ inline void tail_smooth(T *u, const T *f, const T *__restrict__ cx,
+                                   const T *__restrict__ cz, int N, int K,
+                                   int P, T DH2, T relax, int sweeps) {
+  const int half = (K + 1) / 2;
+  for (int s = 0; s < 2 * sweeps; s++) {
+    const int colour = s & 1;
+    for (int idx = threadIdx.x; idx < N * half; idx += blockDim.x) {
+      const int j = idx / half, q = idx - j * half;
+      const int k = 2 * q + ((j + colour) & 1);
+      if (k < 1 || k > K - 2) continue;
+      const int jS = (j == 0) ? 1 : j - 1;
+      const int jN = (j == N - 1) ? N - 2 : j + 1;
+      const size_t g = (size_t)j * P + k;
+      const int c = j * K + k;
+      u[c] = rbgs_update<T, EXACT>(cx[g], cx[g + 1], cz[g], cz[g + P], u[c - 1],
+                                   u[c + 1], u[jS * K + k], u[jN * K + k], u[c],
+                                   DH2 * f[c], true, relax);
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T, bool EXACT, bool FULL_TOP>
+__global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  T *base = reinterpret_cast<T *>(smem);
+  T *U[6], *F[6];
+  size_t off = 0;
+  for (int l = 0; l < a.nlev; l++) {
+    const int n = a.lv[l].N * a.lv[l].K;
+    U[l] = base + off;
+    F[l] = base + off + n;
+    off += 2 * (size_t)n;
+  }
+  T *Rs = base + off;  // residual scratch (entry level size)
+  {
+    const TailLevel &L0 = a.lv[0];
+    const T *fin = reinterpret_cast<const T *>(a.f_in);
+    for (int idx = threadIdx.x; idx < L0.N * L0.K; idx += blockDim.x) {
+      const int j = idx / L0.K, k = idx - j * L0.K;
+      U[0][idx] = T(0);
+      F[0][idx] = (k >= 1 && k <= L0.K - 2) ? fin[(size_t)j * L0.P + k] : T(0);
+    }
+  }
+  __syncthreads();
+  const T relax = (T)a.relax;
+  for (int l = 0; l < a.nlev; l++) {
+    const TailLevel &L = a.lv[l];
+    const T *cx = reinterpret_cast<const T *>(L.cx);
+    const T *cz = reinterpret_cast<const T *>(L.cz);
+    const T DH2 = (T)L.DH2;
+    if (l == a.nlev - 1) {
+      tail_smooth<T, EXACT>(U[l], F[l], cx, cz, L.N, L.K, L.P, DH2, relax,
+                            a.nu_coarse);
+      break;
+    }
+    tail_smooth<T, EXACT>(U[l], F[l], cx, cz, L.N, L.K, L.P, DH2, relax, a.nu_pre);
+    // residual
+    for (int idx = threadIdx.x; idx < L.N * L.K; idx += blockDim.x) {
+      const int j = idx / L.K, k = idx - j * L.K;
+      T r = T(0);
+      if (k >= 1 && k <= L.K - 2) {
+        const int jS = (j == 0) ? 1 : j - 1;
+        const int jN = (j == L.N - 1) ? L.N - 2 : j + 1;
+        const size_t g = (size_t)j * L.P + k;
+        const T m1 = cx[g], m2 = cx[g + 1], m3 = cz[g], m4 = cz[g + L.P];
+        const T *u = U[l];
+        r = F[l][idx] - (m1 * u[idx - 1] + m2 * u[idx + 1] + m3 * u[jS * L.K + k] +
+                         m4 * u[jN * L.K + k] - (m1 + m2 + m3 + m4) * u[idx]) /
+                            DH2;
+      }
+      Rs[idx] = r;
+    }
+    __syncthreads();
+    // restriction to level l+1, zero correction
+    const TailLevel &C = a.lv[l + 1];
+    for (int idx = threadIdx.x; idx < C.N * C.K; idx += blockDim.x) {
+      const int j = idx / C.K, k = idx - j * C.K;
+      T v = T(0);
+      if (k >= 1 && k <= C.K - 2) {
+        const int jS = (j == 0) ? 1 : 2 * j - 1;
+        const int jN = (j == C.N - 1) ? 2 * j - 1 : 2 * j + 1;
+        const T *rc = Rs + (2 * j) * L.K;
+        const T *rs = Rs + jS * L.K;
+        const T *rn = Rs + jN * L.K;
+        const int kf = 2 * k;
+        v = T(0.0625) *
+            (T(4.0) * rc[kf] + T(2.0) * rc[kf - 1] + T(2.0) * rc[kf + 1] +
+             T(2.0) * rs[kf] + T(2.0) * rn[kf] + rs[kf - 1] + rs[kf + 1] +
+             rn[kf - 1] + rn[kf + 1]);
+      }
+      F[l + 1][idx] = v;
+      U[l + 1][idx] = T(0);
+    }
+    __syncthreads();
+  }
+  for (int l = a.nlev - 2; l >= 0; l--) {
+    const TailLevel &L = a.lv[l];
+    const TailLevel &C = a.lv[l + 1];
+    // prolongation: one thread per coarse cell (disjoint fine points)
+    const int jmax = FULL_TOP ? C.N : C.N - 1;
+    for (int idx = threadIdx.x; idx < jmax * (C.K - 1); idx += blockDim.x) {
+      const int j = idx / (C.K - 1), k = idx - j * (C.K - 1);
+      const T *ec = U[l + 1] + j * C.K + k;
+      T *uf = U[l] + (2 * j) * L.K + 2 * k;
+      const T e00 = ec[0], e01 = ec[1];
+      uf[0] += e00;
+      uf[1] += T(0.5) * (e00 + e01);
+      if (!FULL_TOP || j < C.N - 1) {
+        const T e10 = ec[C.K], e11 = ec[C.K + 1];
+        T *uf1 = uf + L.K;
+        uf1[0] += T(0.5) * (e00 + e10);
+        uf1[1] += T(0.25) * (e00 + e01 + e10 + e11);
+      }
+    }
+    __syncthreads();
+    tail_smooth<T, EXACT>(U[l], F[l], reinterpret_cast<const T *>(L.cx),
+                          reinterpret_cast<const T *>(L.cz), L.N, L.K, L.P,
+                          (T)L.DH2, relax, a.nu_post);
+  }
+  {
+    const TailLevel &L0 = a.lv[0];
+    T *eo = reinterpret_cast<T *>(a.e_out);
+    for (int idx = threadIdx.x; idx < L0.N * L0.K; idx += blockDim.x) {
+      const int j = idx / L0.K, k = idx - j * L0.K;
+      eo[(size_t)j * L0.P + k] = U[0][idx];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+constexpr int TAIL_MAX_N = 65;
+
+static size_t tile_lds_bytes(int TX, int TY, int H, bool has_f, size_t esz,
+                             bool down) {
+  const size_t R = TY + 2 * H, C = TX + 2 * H;
+  size_t words = 3 * R * C + C;            // su, scx, scz (+1 row)
+  if (has_f || down) words += R * C;       // sf, or residual scratch
+  return words * esz;
+}
+
+// First level handled by the tail (or L if the coarsest level is too big).
+int tail_entry_level(const qdeq_ctx *c) {
+  for (int l = 0; l < c->L; l++)
+    if (c->lv[l].N <= TAIL_MAX_N && c->lv[l].K <= TAIL_MAX_N && l > 0) {
+      if (c->L - l > 6) return c->L;
+      return l;
+    }
+  return c->L;
+}
+
+template <typename T>
+static size_t tail_lds_bytes(const qdeq_ctx *c, int lt) {
+  size_t n = 0;
+  for (int l = lt; l < c->L; l++) n += 2 * (size_t)c->lv[l].N * c->lv[l].K;
+  n += (size_t)c->lv[lt].N * c->lv[lt].K;
+  return n * sizeof(T);
+}
+
+bool tail_fits(const qdeq_ctx *c, bool fp64) {
+  const int lt = tail_entry_level(c);
+  if (lt >= c->L) return false;
+  const size_t b = fp64 ? tail_lds_bytes<double>(c, lt) : tail_lds_bytes<float>(c, lt);
+  return b + 4096 <= c->lds_max;
+}
+
+template <typename T>
+static void tile_geometry(const qdeq_ctx *c, int level, int *TX, int *TY) {
+  *TX = c->tile_tx;
+  *TY = c->tile_ty;
+  (void)level;
+}
+
+// Largest dynamic LDS the tile kernels of this context would request.
+bool tiles_fit(const qdeq_ctx *c, int tx, int ty) {
+  const bool fp64 = c->cfg.precision != QDEQ_PREC_FP32_IR;
+  const size_t esz = fp64 ? 8 : 4;
+  const size_t down = tile_lds_bytes(tx, ty, 2 * c->cfg.n_pre + 2, true, esz, true);
+  const size_t up = tile_lds_bytes(tx, ty, 2 * c->cfg.n_post + 1, true, esz, false);
+  return down + 4096 <= c->lds_max && up + 4096 <= c->lds_max;
+}
+
+template <typename T, bool EXACT>
+static void launch_down(qdeq_ctx *c, LevelBufs<T> *B, int l, bool has_f,
+                        bool zero_u) {
+  const LevelDims &d = c->lv[l];
+  const LevelDims &dc = c->lv[l + 1];
+  int TX, TY;
+  tile_geometry<T>(c, l, &TX, &TY);
+  const int nu = c->cfg.n_pre;
+  const int H = 2 * nu + 2;
+  const dim3 grd((d.K + TX - 1) / TX, (d.N + TY - 1) / TY);
+  const size_t lds = tile_lds_bytes(TX, TY, H, has_f, sizeof(T), true);
+  const T DH2 = (T)(d.DH * d.DH), relax = (T)c->cfg.relax;
+  if (has_f)
+    hipLaunchKernelGGL((k_down<T, true, EXACT>), grd, dim3(TILE_THREADS), lds,
+                       c->stream, B[l].u, B[l].r, B[l].f, B[l].cx, B[l].cz, d.N,
+                       d.K, d.P, DH2, relax, nu, zero_u ? 1 : 0, B[l + 1].f,
+                       dc.N, dc.K, dc.P, TX, TY);
+  else
+    hipLaunchKernelGGL((k_down<T, false, EXACT>), grd, dim3(TILE_THREADS), lds,
+                       c->stream, B[l].u, B[l].r, B[l].f, B[l].cx, B[l].cz, d.N,
+                       d.K, d.P, DH2, relax, nu, zero_u ? 1 : 0, B[l + 1].f,
+                       dc.N, dc.K, dc.P, TX, TY);
+}
+
+template <typename T, bool EXACT, bool HAS_F, int MODE>
+static void launch_up_t(qdeq_ctx *c, LevelBufs<T> *B, int l, double *partial,
+                        double *u64, int *nblocks) {
+  const LevelDims &d = c->lv[l];
+  const LevelDims &dc = c->lv[l + 1];
+  int TX, TY;
+  tile_geometry<T>(c, l, &TX, &TY);
+  const int nu = c->cfg.n_post;
+  const int H = 2 * nu + (MODE == UP_NORM ? 1 : 0);
+  const dim3 grd((d.K + TX - 1) / TX, (d.N + TY - 1) / TY);
+  if (nblocks) *nblocks = (int)(grd.x * grd.y);
+  const size_t lds = tile_lds_bytes(TX, TY, H, HAS_F, sizeof(T), false);
+  const T DH2 = (T)(d.DH * d.DH), relax = (T)c->cfg.relax;
+  hipLaunchKernelGGL((k_up<T, HAS_F, EXACT, true, MODE>), grd, dim3(TILE_THREADS),
+                     lds, c->stream, B[l].r, B[l].u, B[l].f, B[l].cx, B[l].cz,
+                     d.N, d.K, d.P, DH2, relax, nu, B[l + 1].u, dc.N, dc.K, dc.P,
+                     TX, TY, partial, u64);
+}
+
+template <typename T, bool EXACT>
+static void launch_tail(qdeq_ctx *c, LevelBufs<T> *B, int lt) {
+  TailArgs a{};
+  a.nlev = c->L - lt;
+  for (int i = 0; i < a.nlev; i++) {
+    const LevelDims &d = c->lv[lt + i];
+    a.lv[i] = TailLevel{B[lt + i].cx, B[lt + i].cz, d.N, d.K, d.P, d.DH * d.DH};
+  }
+  a.f_in = B[lt].f;
+  a.e_out = B[lt].u;
+  a.nu_pre = c->cfg.n_pre;
+  a.nu_post = c->cfg.n_post;
+  a.nu_coarse = c->cfg.n_coarse;
+  a.relax = c->cfg.relax;
+  hipLaunchKernelGGL((k_tail<T, EXACT, true>), dim3(1), dim3(TAIL_THREADS),
+                     tail_lds_bytes<T>(c, lt), c->stream, a);
+}
+
+// Fast V-cycle for the FP64 solve (level 0 = the displacement, f = 0) or
+// the FP32 correction of iterative refinement (level 0 = correction e,
+// f = r32; the last up kernel adds e into the FP64 displacement).  The
+// level-0 norm partials land in `partial` (FP64) and the count of partials
+// in *nparts.
+template <typename T, bool EXACT>
+static void vcycle_tiled(qdeq_ctx *c, LevelBufs<T> *B, bool level0_has_f,
+                         double *partial, double *u64, int *nparts) {
+  const int lt = tail_entry_level(c);
+  for (int l = 0; l < lt; l++)
+    launch_down<T, EXACT>(c, B, l, l > 0 || level0_has_f, l > 0 || level0_has_f);
+  launch_tail<T, EXACT>(c, B, lt);
+  for (int l = lt - 1; l >= 1; l--)
+    launch_up_t<T, EXACT, true, UP_STORE>(c, B, l, nullptr, nullptr, nullptr);
+  if (u64)
+    launch_up_t<T, EXACT, true, UP_ADD64>(c, B, 0, nullptr, u64, nullptr);
+  else if (level0_has_f)
+    launch_up_t<T, EXACT, true, UP_STORE>(c, B, 0, nullptr, nullptr, nullptr);
+  else
+    launch_up_t<T, EXACT, false, UP_NORM>(c, B, 0, partial, nullptr, nparts);
+}
+
+void vcycle_tiled_fp64(qdeq_ctx *c, double *partial, int *nparts) {
+  vcycle_tiled<double, true>(c, c->d, false, partial, nullptr, nparts);
+}
+
+void vcycle_tiled_fp32_ir(qdeq_ctx *c) {
+  vcycle_tiled<float, false>(c, c->s, true, nullptr, c->d[0].u, nullptr);
+}
+
+int tiled_setup(qdeq_ctx *c) {
+  // Per-workgroup LDS available to dynamic allocations on this device; the
+  // attribute raises the default cap where the runtime needs it.
+  int dev = 0, optin = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipGetDeviceProperties(&prop, dev) == hipSuccess)
+    c->lds_max = prop.sharedMemPerBlock;
+  if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin,
+                            dev) == hipSuccess &&
+      (size_t)optin > c->lds_max)
+    c->lds_max = (size_t)optin;
+  const int big = (int)c->lds_max;
+#define SETA(kern)                                                       \
+  (void)hipFuncSetAttribute((const void *)(kern),                        \
+                            hipFuncAttributeMaxDynamicSharedMemorySize, big)
+  SETA((k_down<double, true, true>));
+  SETA((k_down<double, false, true>));
+  SETA((k_down<float, true, false>));
+  SETA((k_down<float, false, false>));
+  SETA((k_up<double, true, true, true, UP_STORE>));
+  SETA((k_up<double, false, true, true, UP_NORM>));
+  SETA((k_up<float, true, false, true, UP_STORE>));
+  SETA((k_up<float, true, false, true, UP_ADD64>));
+  SETA((k_tail<double, true, true>));
+  SETA((k_tail<float, false, true>));
+#undef SETA
+  (void)hipGetLastError();
+  if (!tiles_fit(c, c->tile_tx, c->tile_ty)) c->tiled = false;
+  return QDEQ_OK;
+}
+
+}  // namespace qdeq

@@ -81,6 +81,10 @@ struct qdeq_ctx {
   long long vcycles = 0, solves = 0, ir_sweeps = 0;
   double c_stress[4];  // partial1st coefficients (fault.cpp:162)
   std::vector<double> mu_fault;  // nodal mu at the fault column (host copy)
+  // fast cycle: tiled kernels and their interior tile shape (even sizes)
+  bool tiled = true;
+  int tile_tx = 64, tile_ty = 32;
+  size_t lds_max = 65536;  // dynamic LDS per workgroup (queried at creation)
 };
 
 namespace qdeq {
@@ -91,6 +95,15 @@ int set_error(qdeq_ctx *c, int code, const std::string &msg);
 int mg_solve(qdeq_ctx *c, int *vcycles, double *res0, double *res);
 int mg_build_coefficients(qdeq_ctx *c);
 void mg_destroy_graphs(qdeq_ctx *c);
+bool use_tiled(const qdeq_ctx *c);
+
+// tiled fast cycle (mg_tiled.hip)
+int tail_entry_level(const qdeq_ctx *c);
+bool tail_fits(const qdeq_ctx *c, bool fp64);
+void vcycle_tiled_fp64(qdeq_ctx *c, double *partial, int *nparts);
+void vcycle_tiled_fp32_ir(qdeq_ctx *c);
+int tiled_setup(qdeq_ctx *c);
+bool tiles_fit(const qdeq_ctx *c, int tx, int ty);
 
 // medium (medium.hip)
 int build_mu_equi_device(int ni, int nk, const double *mu_dev, double *me_dev,

@@ -37,7 +37,7 @@ EXPORTS = (
     "qdeq_ode_solve", "qdeq_set_boundary", "qdeq_eval_stress",
     "qdeq_nr_search", "qdeq_cal_dpsi", "qdeq_compute_error",
     "qdeq_compute_step_size", "qdeq_rk43_setup", "qdeq_rk43_step",
-    "qdeq_rk43_get_state",
+    "qdeq_rk43_get_state", "qdeq_set_tiles", "qdeq_uses_tiled_cycle",
 )
 
 
@@ -116,6 +116,8 @@ def load(path: str = LIB_PATH):
                                          ctypes.c_double, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_double, _dp]
     L.qdeq_compute_step_size.restype = ctypes.c_double
+    L.qdeq_set_tiles.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.qdeq_uses_tiled_cycle.argtypes = [_vp]
     L.qdeq_rk43_setup.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp]
     L.qdeq_rk43_step.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(StepInfo)]
     L.qdeq_rk43_get_state.argtypes = [_vp, ctypes.POINTER(StepInfo), _dp, _dp,
@@ -222,6 +224,14 @@ class Engine:
 
     def __exit__(self, *a):
         self.close()
+
+    def set_tiles(self, tiled: bool = True, tile_x: int = 64, tile_y: int = 32):
+        """Fast-cycle execution shape (identical results either way)."""
+        self._check(load().qdeq_set_tiles(self.h, 1 if tiled else 0, tile_x, tile_y))
+
+    @property
+    def uses_tiled_cycle(self) -> bool:
+        return bool(load().qdeq_uses_tiled_cycle(self.h))
 
     def mu_equi(self) -> np.ndarray:
         out = np.zeros((2 * self.nz - 1, 2 * self.nx - 1))

@@ -1,0 +1,133 @@
+"""GPU checks of the fast cycle (the engine's production solver).
+
+* FP64 fast cycle on the tiled LDS kernels == the same cycle launched one
+  half-sweep at a time == the oracle's fast cycle, bit for bit.
+* FP32 V-cycles in FP64 iterative refinement reach the 1e-10 relative
+  residual and the FP64 oracle's solution to the accuracy that residual
+  implies; tiled and untiled IR are bit-identical.
+* The RK43 driver in the fast FP64 mode matches the oracle step for step.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from qdeqcycle_fdm_amd import config, lib
+
+pytestmark = pytest.mark.gpu
+
+FAST = dict(cycle=config.CYCLE_FAST, n_pre=2, n_post=2, n_coarse=32)
+
+
+def _perturbed(pb, seed, scale=0.5):
+    rng = np.random.default_rng(seed)
+    U = pb.U0.copy()
+    U[1:-1, 2:-2] += rng.normal(scale=scale, size=(pb.params.nk, pb.params.ni - 2))
+    return U
+
+
+def _fast_oracle_solve(pb, me, U, ncap, tol=1e-300, mode=oracle.ORC_TOL_ABS):
+    p = pb.params
+    mg = oracle.MG(p.ni, p.nk, p.dh, p.n_levels, me)
+    mg.set_cycle(FAST["n_pre"], FAST["n_post"], FAST["n_coarse"], True)
+    V = U.copy()
+    st, vc, r0, r = mg.solve(V, mode, tol, ncap)
+    return V, vc, r0, r
+
+
+def _gpu_solve(pb, me, U, tiled, tile=(64, 16), **kw):
+    with lib.Engine(pb, me, **{**FAST, **kw}) as eng:
+        eng.set_tiles(tiled, *tile)
+        assert eng.uses_tiled_cycle == tiled
+        V = U.copy()
+        try:
+            out = eng.ode_solve(V)
+        except lib.QdeqError as e:
+            if e.status != lib.QDEQ_ERR_MG_NOT_CONVERGED:
+                raise
+            out = None
+    return V, out
+
+
+@pytest.mark.parametrize("n,levels", [(65, 3), (129, 4), (257, 5)])
+@pytest.mark.parametrize("ncap", [1, 3])
+def test_fast_fp64_tiled_equals_untiled_equals_oracle(n, levels, ncap):
+    pb = config.make_bp1_problem(n, basin=True, levels=levels)
+    me = oracle.mu_equi(pb.mu)
+    U = _perturbed(pb, 3)
+    ref, vc, _, _ = _fast_oracle_solve(pb, me, U, ncap)
+    assert vc == ncap
+    untiled, _ = _gpu_solve(pb, me, U, False, tol=1e-300, max_vcycles=ncap)
+    np.testing.assert_array_equal(untiled, ref)
+    for tile in ((64, 16), (32, 8), (128, 4)):
+        tiled, _ = _gpu_solve(pb, me, U, True, tile=tile, tol=1e-300, max_vcycles=ncap)
+        np.testing.assert_array_equal(tiled, ref, err_msg=str(tile))
+
+
+def test_fast_fp64_relative_tolerance_solve():
+    pb = config.make_bp1_problem(257, basin=True, tol_mode=config.TOL_REL, tol=1e-10)
+    me = oracle.mu_equi(pb.mu)
+    U = _perturbed(pb, 4)
+    ref, vc_ref, r0_ref, r_ref = _fast_oracle_solve(pb, me, U, 200, 1e-10,
+                                                    oracle.ORC_TOL_REL)
+    got, (vc, r0, r) = _gpu_solve(pb, me, U, True)
+    assert vc == vc_ref
+    assert r <= 1e-10 * r0
+    np.testing.assert_array_equal(got, ref)
+    np.testing.assert_allclose(r0, r0_ref, rtol=1e-13)
+
+
+@pytest.mark.parametrize("tiled", [False, True])
+def test_ir_fp32_reaches_fp64_tolerance(tiled):
+    """FP32 correction cycles in FP64 refinement: relative residual <= 1e-10
+    and the solution within the error that residual allows of the FP64
+    oracle's converged solution."""
+    pb = config.make_bp1_problem(257, basin=True, tol_mode=config.TOL_REL, tol=1e-10)
+    me = oracle.mu_equi(pb.mu)
+    U = _perturbed(pb, 5)
+    ref, _, _, _ = _fast_oracle_solve(pb, me, U, 400, 1e-13, oracle.ORC_TOL_REL)
+    fp64, _ = _gpu_solve(pb, me, U, True)          # FP64 fast cycle, same tolerance
+    got, (vc, r0, r) = _gpu_solve(pb, me, U, tiled, tile=(64, 32),
+                                  precision=config.PREC_FP32_IR)
+    assert r <= 1e-10 * r0
+    assert vc < 40
+    scale = np.max(np.abs(ref[1:-1, 1:-1]))
+    err_ir = np.max(np.abs(got[1:-1, 1:-1] - ref[1:-1, 1:-1]))
+    err_64 = np.max(np.abs(fp64[1:-1, 1:-1] - ref[1:-1, 1:-1]))
+    # a 1e-10 relative residual bounds the error only through the operator's
+    # condition number: both solves sit at that level, far below 1e-6
+    assert err_ir <= 1e-6 * scale
+    assert err_ir <= 10.0 * max(err_64, 1e-12 * scale)
+
+
+def test_ir_tiled_equals_untiled():
+    pb = config.make_bp1_problem(129, basin=True, tol_mode=config.TOL_REL, tol=1e-10)
+    me = oracle.mu_equi(pb.mu)
+    U = _perturbed(pb, 6)
+    a, oa = _gpu_solve(pb, me, U, False, precision=config.PREC_FP32_IR)
+    b, ob = _gpu_solve(pb, me, U, True, tile=(64, 32), precision=config.PREC_FP32_IR)
+    assert oa[0] == ob[0]
+    np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("n,basin,levels,steps", [(129, False, 4, 60), (65, True, 3, 40)])
+def test_rk43_fast_fp64_matches_oracle(n, basin, levels, steps):
+    pb = config.make_bp1_problem(n, basin=basin, levels=levels,
+                                 tol_mode=config.TOL_REL, tol=1e-10)
+    me = oracle.mu_equi(pb.mu)
+    oracle.set_libm(False)
+    sim = oracle.Sim(pb, me, cycle=1, n_pre=FAST["n_pre"], n_post=FAST["n_post"],
+                     n_coarse=FAST["n_coarse"])
+    sim.init()
+    with lib.Engine(pb, me, **FAST) as eng:
+        assert eng.uses_tiled_cycle
+        eng.rk43_setup()
+        for s in range(steps):
+            sim.step()
+            info = eng.rk43_step(1)
+            ref = sim.scalars()
+            for key in ("t", "dt", "totErr", "V_max", "attempts", "vcycles"):
+                assert info[key] == ref[key], (s, key, info[key], ref[key])
+        st = eng.state()
+    ar = sim.arrays()
+    for key in ("slip", "psi", "V", "U"):
+        np.testing.assert_array_equal(st[key], ar[key], err_msg=key)
