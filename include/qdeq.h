@@ -58,7 +58,9 @@ typedef struct qdeq_config {
   int ni, nk;        /* interior nodes in x and z (SetParams.cpp:21-22) */
   double dh;         /* grid spacing in m (SetParams.cpp:23) */
   int n_levels;      /* multigrid levels (odeSolver.cpp:3) */
-  int n_pre, n_post, n_coarse; /* sweeps (odeSolver.cpp:4: 32 each) */
+  int n_pre, n_post, n_coarse; /* sweeps (odeSolver.cpp:4: 32 each); with the
+                                  fast cycle n_coarse = 0 solves the coarsest
+                                  level exactly (at most 1024 unknowns) */
   double relax;      /* RBGS relaxation (odeSolver.cpp:2) */
   int cycle;         /* QDEQ_CYCLE_* */
   int precision;     /* QDEQ_PREC_* */
@@ -103,6 +105,19 @@ int qdeq_destroy(qdeq_ctx *ctx);
 int qdeq_set_tiles(qdeq_ctx *ctx, int tiled, int tile_x, int tile_y);
 /* 1 if solves of ctx currently run the fast cycle on the tiled kernels. */
 int qdeq_uses_tiled_cycle(const qdeq_ctx *ctx);
+
+/* Measurement hooks (no reference counterpart).  qdeq_set_kernel_timing
+ * brackets one kernel of the tiled fast cycle with HIP events on the
+ * context's stream (the captured V-cycle is split into three graphs around
+ * it, the events recorded between their launches) and accumulates its
+ * duration over every following V-cycle; qdeq_get_kernel_timing reads the
+ * total and the count.  qdeq_copy_probe times `reps` launches of a plain
+ * device copy of n_doubles doubles (bandwidth and counter calibration). */
+enum { QDEQ_TIME_OFF = 0, QDEQ_TIME_UP0 = 1, QDEQ_TIME_TAIL = 2,
+       QDEQ_TIME_DOWN1 = 3 };
+int qdeq_set_kernel_timing(qdeq_ctx *ctx, int which);
+int qdeq_get_kernel_timing(qdeq_ctx *ctx, double *total_ms, long long *calls);
+int qdeq_copy_probe(qdeq_ctx *ctx, long long n_doubles, int reps, double *avg_ms);
 
 /* Message of the last failing call on ctx ("" if none). */
 const char *qdeq_last_error(const qdeq_ctx *ctx);

@@ -112,6 +112,10 @@ def lib():
         L.orc_sim_destroy.argtypes = [ctypes.c_void_p]
         L.orc_sim_init.argtypes = [ctypes.c_void_p]
         L.orc_sim_init.restype = ctypes.c_int
+        L.orc_sim_set_state.argtypes = [ctypes.c_void_p, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_int,
+                                        ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
+        L.orc_sim_set_state.restype = ctypes.c_int
         L.orc_sim_step.argtypes = [ctypes.c_void_p]
         L.orc_sim_step.restype = ctypes.c_int
         L.orc_sim_scalars.argtypes = [ctypes.c_void_p, _dp]
@@ -301,6 +305,14 @@ class Sim:
         st = lib().orc_sim_init(self.h)
         if st != ORC_OK:
             raise RuntimeError(f"oracle init failed: status {st}")
+
+    def set_state(self, t, dt, it, errA0, errA1, U, slip, psi):
+        """Restart from a saved state (recomputes stage 1)."""
+        self._keep = [_c(U), _c(slip), _c(psi)]
+        st = lib().orc_sim_set_state(self.h, t, dt, int(it), errA0, errA1,
+                                     *[_p(a) for a in self._keep])
+        if st != ORC_OK:
+            raise RuntimeError(f"oracle restart failed: status {st}")
 
     def step(self):
         st = lib().orc_sim_step(self.h)

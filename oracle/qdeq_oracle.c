@@ -406,6 +406,8 @@ double orc_nrm2(const double *x, long n) {
 struct orc_mg {
   int ni, nk, L, nu;
   int n_pre, n_post, n_coarse, full_top;  /* cycle shape (reference: nu,nu,nu,0) */
+  double *cinv;                  /* fast cycle, n_coarse == 0: inverse of the */
+  int cn;                        /* coarsest operator (row-major), unknowns   */
   double dh, relax;
   int N[ORC_MAX_LEVELS], K[ORC_MAX_LEVELS], FC[ORC_MAX_LEVELS];
   double DH[ORC_MAX_LEVELS];
@@ -448,7 +450,87 @@ void orc_mg_destroy(orc_mg *mg) {
     free(mg->res[l]); free(mg->res2[l]); free(mg->err2[l]);
   }
   free(mg->mu); free(mg->u); free(mg->f); free(mg->delta); free(mg->tmp);
+  free(mg->cinv);
   free(mg);
+}
+
+/* The engine's fast cycle may solve the coarsest level exactly (no
+ * reference counterpart): B = DH^2 L assembled on the unknowns
+ * j*(K-2) + k-1 from the same coefficients the sweeps use, inverted by
+ * Gauss-Jordan elimination without pivoting, applied as e = B^{-1}(DH^2 f)
+ * with each row summed in column order. */
+static void orc_coarse_inverse(orc_mg *mg) {
+  const int l = mg->L - 1;
+  const long N = mg->N[l], K = mg->K[l], W = K - 2, n = N * W;
+  const long W2 = 2L * (mg->ni + 2) - 1;
+  double *B = (double *)calloc((size_t)(n * n), sizeof(double));
+  double *X = (double *)calloc((size_t)(n * n), sizeof(double));
+  for (long j = 0; j < N; j++)
+    for (long k = 1; k <= K - 2; k++) {
+      long i1, i2, i3, i4;
+      orc_mu_index(l, mg->FC[l], W2, j, k, &i1, &i2, &i3, &i4);
+      const double m1 = mg->mu[i1], m2 = mg->mu[i2];
+      double m3, m4;
+      if (j == 0) {
+        m4 = mg->mu[i4];
+        m3 = (l == 0) ? mg->mu[i3] : m4;
+      } else if (j == N - 1) {
+        m3 = mg->mu[i3];
+        m4 = (l == 0) ? mg->mu[i4] : m3;
+      } else {
+        m3 = mg->mu[i3];
+        m4 = mg->mu[i4];
+      }
+      const long jS = (j == 0) ? 1 : j - 1, jN = (j == N - 1) ? N - 2 : j + 1;
+      const long row = j * W + (k - 1);
+      double *Br = B + row * n;
+      if (k - 1 >= 1) Br[j * W + (k - 2)] += m1;
+      if (k + 1 <= K - 2) Br[j * W + k] += m2;
+      Br[jS * W + (k - 1)] += m3;
+      Br[jN * W + (k - 1)] += m4;
+      Br[row] -= (m1 + m2 + m3 + m4);
+    }
+  for (long i = 0; i < n; i++) X[i * n + i] = 1.0;
+  for (long p = 0; p < n; p++) {
+    double *Ap = B + p * n, *Xp = X + p * n;
+    const double piv = Ap[p];
+    for (long c = 0; c < n; c++) {
+      Ap[c] = Ap[c] / piv;
+      Xp[c] = Xp[c] / piv;
+    }
+    for (long r = 0; r < n; r++) {
+      if (r == p) continue;
+      double *Ar = B + r * n, *Xr = X + r * n;
+      const double f = Ar[p];
+      if (f == 0.0) continue;
+      for (long c = 0; c < n; c++) {
+        Ar[c] = Ar[c] - f * Ap[c];
+        Xr[c] = Xr[c] - f * Xp[c];
+      }
+    }
+  }
+  free(B);
+  free(mg->cinv);
+  mg->cinv = X;
+  mg->cn = (int)n;
+}
+
+static void orc_coarse_direct(orc_mg *mg, double *u, const double *f,
+                              int level) {
+  const long K = mg->K[level], W = K - 2, n = mg->cn;
+  const double DH2 = mg->DH[level] * mg->DH[level];
+  double *g = (double *)malloc(sizeof(double) * (size_t)n);
+  for (long i = 0; i < n; i++) {
+    const long j = i / W, k = i - j * W + 1;
+    g[i] = DH2 * f[j * K + k];
+  }
+  for (long row = 0; row < n; row++) {
+    double acc = 0.0;
+    for (long col = 0; col < n; col++) acc = acc + mg->cinv[row * n + col] * g[col];
+    const long j = row / W, k = row - j * W + 1;
+    u[j * K + k] = acc;
+  }
+  free(g);
 }
 
 void orc_mg_set_cycle(orc_mg *mg, int n_pre, int n_post, int n_coarse,
@@ -457,6 +539,10 @@ void orc_mg_set_cycle(orc_mg *mg, int n_pre, int n_post, int n_coarse,
   mg->n_post = n_post;
   mg->n_coarse = n_coarse;
   mg->full_top = full_top;
+  free(mg->cinv);
+  mg->cinv = NULL;
+  mg->cn = 0;
+  if (full_top && n_coarse == 0) orc_coarse_inverse(mg);
 }
 
 /* smooth, odeSolver.cpp:741-755: sweeps x (red, black). */
@@ -472,7 +558,10 @@ static void orc_smooth(orc_mg *mg, double *u, const double *f, int level,
 static void orc_vcycle_level(orc_mg *mg, double *u, const double *f,
                              int level) {
   if (level >= mg->L - 1) {
-    orc_smooth(mg, u, f, level, mg->n_coarse);
+    if (mg->cinv)
+      orc_coarse_direct(mg, u, f, level);
+    else
+      orc_smooth(mg, u, f, level, mg->n_coarse);
     return;
   }
   orc_smooth(mg, u, f, level, mg->n_pre);
@@ -978,6 +1067,28 @@ static int orc_stage(orc_sim *s, const double *ys, const double *yp,
   if (st != ORC_OK) return st;
   orc_cal_dpsi(p->nk, yp, fV, s->b, p->f0, p->V0, p->L, fd);
   return ORC_OK;
+}
+
+/* Restart from a saved state: t, dt, step count, controller history and
+ * the committed (U, slip, psi); stage 1 is recomputed as after an accepted
+ * step (odeSolver.cpp:1615-1619). */
+int orc_sim_set_state(orc_sim *s, double t, double dt, int it, double errA0,
+                      double errA1, const double *U, const double *slip,
+                      const double *psi) {
+  const size_t ng = (size_t)(s->nx * s->nz), nz = (size_t)s->nz;
+  s->t = t;
+  s->dt = dt;
+  s->it = it;
+  s->errA[0] = errA0;
+  s->errA[1] = errA1;
+  memcpy(s->U, U, ng * sizeof(double));
+  memcpy(s->slip, slip, nz * sizeof(double));
+  memcpy(s->psi, psi, nz * sizeof(double));
+  for (int i = 1; i <= 6; i++) {
+    memset(s->fV[i], 0, nz * sizeof(double));
+    memset(s->fd[i], 0, nz * sizeof(double));
+  }
+  return orc_sim_init(s);
 }
 
 int orc_sim_init(orc_sim *s) {

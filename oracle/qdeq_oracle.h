@@ -123,6 +123,9 @@ orc_sim *orc_sim_create(const orc_params *p, const double *mu,
                         const double *tau_n0, const double *U0);
 void orc_sim_destroy(orc_sim *s);
 int orc_sim_init(orc_sim *s);          /* odeSolver.cpp:1428-1437 */
+int orc_sim_set_state(orc_sim *s, double t, double dt, int it, double errA0,
+                      double errA1, const double *U, const double *slip,
+                      const double *psi);
 int orc_sim_step(orc_sim *s);          /* one accepted step, 1462-1633 */
 /* scalars: t, dt, it, errA0, errA1, totErr, attempts, vcycles(total),
  * solves(total), V_max, Ts_max */

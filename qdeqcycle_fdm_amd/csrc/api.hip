@@ -79,6 +79,8 @@ static void free_ctx(qdeq_ctx *c) {
     (void)hipFree(c->s[l].cx); (void)hipFree(c->s[l].cz);
   }
   (void)hipFree(c->mu_equi);
+  (void)hipFree(c->cinv64);
+  (void)hipFree(c->cinv32);
   (void)hipFree(c->partial);
   (void)hipFree(c->scal);
   (void)hipFree(c->flags);
@@ -87,6 +89,8 @@ static void free_ctx(qdeq_ctx *c) {
   (void)hipFree(c->fb.base);
   if (c->scal_host) (void)hipHostFree(c->scal_host);
   if (c->flags_host) (void)hipHostFree(c->flags_host);
+  if (c->ev_t0) (void)hipEventDestroy(c->ev_t0);
+  if (c->ev_t1) (void)hipEventDestroy(c->ev_t1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -109,7 +113,9 @@ int qdeq_create(const qdeq_config *cfg, const double *mu,
   *out = nullptr;
   if (cfg->ni < 5 || cfg->nk < 5 || cfg->dh <= 0 || cfg->n_levels < 1 ||
       cfg->n_levels > QDEQ_MAX_LEVELS || cfg->n_pre < 0 || cfg->n_post < 0 ||
-      cfg->n_coarse < 1 || cfg->max_vcycles < 1 || cfg->tol <= 0)
+      cfg->n_coarse < 0 ||
+      (cfg->n_coarse == 0 && cfg->cycle != QDEQ_CYCLE_FAST) ||
+      cfg->max_vcycles < 1 || cfg->tol <= 0)
     return set_error(nullptr, QDEQ_ERR_ARG, "qdeq_create: invalid config");
   qdeq_ctx *c = new qdeq_ctx();
   c->cfg = *cfg;
@@ -211,7 +217,8 @@ int qdeq_create(const qdeq_config *cfg, const double *mu,
                               hipMemcpyHostToDevice, c->stream));
     CREATE_HIP(hipStreamSynchronize(c->stream));
   }
-  if (mg_build_coefficients(c) != QDEQ_OK || tiled_setup(c) != QDEQ_OK) {
+  if (mg_build_coefficients(c) != QDEQ_OK || build_coarse_inverse(c) != QDEQ_OK ||
+      tiled_setup(c) != QDEQ_OK) {
     set_error(nullptr, QDEQ_ERR_HIP, c->err);
     free_ctx(c);
     return QDEQ_ERR_HIP;
@@ -240,6 +247,63 @@ int qdeq_set_tiles(qdeq_ctx *c, int tiled, int tile_x, int tile_y) {
 }
 
 int qdeq_uses_tiled_cycle(const qdeq_ctx *c) { return c && use_tiled(c) ? 1 : 0; }
+
+int qdeq_set_kernel_timing(qdeq_ctx *c, int which) {
+  if (!c || which < 0 || which > QDEQ_TIME_DOWN1)
+    return set_error(c, QDEQ_ERR_ARG, "bad timing selector");
+  if (which && !c->ev_t0) {
+    QDEQ_HIP(c, hipEventCreate(&c->ev_t0));
+    QDEQ_HIP(c, hipEventCreate(&c->ev_t1));
+  }
+  c->timing = which;
+  c->timed_ms = 0.0;
+  c->timed_calls = 0;
+  mg_destroy_graphs(c);
+  return QDEQ_OK;
+}
+
+int qdeq_get_kernel_timing(qdeq_ctx *c, double *total_ms, long long *calls) {
+  if (!c) return set_error(c, QDEQ_ERR_ARG, "null argument");
+  if (total_ms) *total_ms = c->timed_ms;
+  if (calls) *calls = c->timed_calls;
+  return QDEQ_OK;
+}
+
+__global__ void k_copy_probe(const double *__restrict__ a, double *__restrict__ b,
+                             size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) b[i] = a[i];
+}
+
+int qdeq_copy_probe(qdeq_ctx *c, long long n_doubles, int reps, double *avg_ms) {
+  if (!c || n_doubles <= 0 || reps <= 0)
+    return set_error(c, QDEQ_ERR_ARG, "bad copy probe size");
+  double *a = nullptr, *b = nullptr;
+  hipEvent_t e0, e1;
+  QDEQ_HIP(c, hipMalloc(&a, (size_t)n_doubles * sizeof(double)));
+  QDEQ_HIP(c, hipMalloc(&b, (size_t)n_doubles * sizeof(double)));
+  QDEQ_HIP(c, hipMemsetAsync(a, 0, (size_t)n_doubles * sizeof(double), c->stream));
+  QDEQ_HIP(c, hipEventCreate(&e0));
+  QDEQ_HIP(c, hipEventCreate(&e1));
+  const unsigned blocks = 256 * 8;
+  hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, c->stream, a, b,
+                     (size_t)n_doubles);
+  QDEQ_HIP(c, hipEventRecord(e0, c->stream));
+  for (int r = 0; r < reps; r++)
+    hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, c->stream, a, b,
+                       (size_t)n_doubles);
+  QDEQ_HIP(c, hipEventRecord(e1, c->stream));
+  QDEQ_HIP(c, hipEventSynchronize(e1));
+  float ms = 0.0f;
+  QDEQ_HIP(c, hipEventElapsedTime(&ms, e0, e1));
+  if (avg_ms) *avg_ms = ms / reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return QDEQ_OK;
+}
 
 int qdeq_get_mu_equi(qdeq_ctx *c, double *me, int where) {
   if (!c || !me) return set_error(c, QDEQ_ERR_ARG, "null argument");

@@ -280,6 +280,13 @@ static void smooth(qdeq_ctx *c, LevelBufs<T> &B, const LevelDims &d, int sweeps,
   }
 }
 
+static void coarse_direct(qdeq_ctx *c, LevelBufs<double> &B) {
+  launch_coarse_direct_fp64(c, B.u, B.f);
+}
+static void coarse_direct(qdeq_ctx *c, LevelBufs<float> &B) {
+  launch_coarse_direct_fp32(c, B.u, B.f);
+}
+
 // vcycle (odeSolver.cpp:1101-1147) on buffers B (FP64 or FP32 hierarchy).
 template <typename T, bool EXACT>
 static void vcycle(qdeq_ctx *c, LevelBufs<T> *B, int level, bool level0_has_f) {
@@ -287,7 +294,10 @@ static void vcycle(qdeq_ctx *c, LevelBufs<T> *B, int level, bool level0_has_f) {
   const bool ref = c->cfg.cycle == QDEQ_CYCLE_REFERENCE;
   const bool has_f = level > 0 || level0_has_f;
   if (level >= c->L - 1) {
-    smooth<T, EXACT>(c, B[level], d, c->cfg.n_coarse, has_f);
+    if (!ref && c->coarse_n > 0)
+      coarse_direct(c, B[level]);
+    else
+      smooth<T, EXACT>(c, B[level], d, c->cfg.n_coarse, has_f);
     return;
   }
   smooth<T, EXACT>(c, B[level], d, c->cfg.n_pre, has_f);
@@ -356,9 +366,12 @@ bool use_tiled(const qdeq_ctx *c) {
 // the oracle); the FP32 correction cycle uses a plain division.
 static void launch_iteration(qdeq_ctx *c) {
   const bool tiled = use_tiled(c);
+  c->cur_phase = PH_PRE;
   if (c->cfg.precision == QDEQ_PREC_FP32_IR) {
     if (tiled) {
       vcycle_tiled_fp32_ir(c);
+      c->cur_phase = PH_POST;
+      if (!emit_now(c)) return;
     } else {
       vcycle<float, false>(c, c->s, 0, true);
       const LevelDims &d = c->lv[0];
@@ -369,6 +382,8 @@ static void launch_iteration(qdeq_ctx *c) {
   } else if (tiled) {
     int nparts = 0;
     vcycle_tiled_fp64(c, c->partial, &nparts);
+    c->cur_phase = PH_POST;
+    if (!emit_now(c)) return;
     hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(NORM_THREADS), 0, c->stream,
                        c->partial, nparts, c->scal + SC_NORM);
   } else {
@@ -380,23 +395,38 @@ static void launch_iteration(qdeq_ctx *c) {
 void mg_destroy_graphs(qdeq_ctx *c) {
   if (c->g_cycle) (void)hipGraphExecDestroy(c->g_cycle);
   c->g_cycle = nullptr;
+  for (auto &g : c->g_part) {
+    if (g) (void)hipGraphExecDestroy(g);
+    g = nullptr;
+  }
+  c->g_timed = false;
   c->g_cycle_mode = -1;
 }
 
 static int ensure_graph(qdeq_ctx *c) {
   const int mode = c->cfg.precision * 2 + c->cfg.cycle + (c->tiled ? 4 : 0) +
-                   8 * (c->tile_tx * 1024 + c->tile_ty);
+                   8 * c->timing + 64 * (c->tile_tx * 1024 + c->tile_ty);
   if (c->g_cycle && c->g_cycle_mode == mode) return QDEQ_OK;
   mg_destroy_graphs(c);
-  hipGraph_t g = nullptr;
-  QDEQ_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  launch_iteration(c);
-  QDEQ_HIP(c, hipStreamEndCapture(c->stream, &g));
-  hipError_t e = hipGraphInstantiate(&c->g_cycle, g, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(g);
-  if (e != hipSuccess)
-    return set_error(c, QDEQ_ERR_HIP,
-                     std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  const bool split = c->timing != 0 && use_tiled(c);
+  for (int ph = split ? PH_PRE : PH_ALL; ph <= (split ? PH_POST : PH_ALL); ph++) {
+    hipGraph_t g = nullptr;
+    c->emit_phase = ph;
+    hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      launch_iteration(c);
+      e = hipStreamEndCapture(c->stream, &g);
+    }
+    c->emit_phase = PH_ALL;
+    if (e == hipSuccess)
+      e = hipGraphInstantiate(split ? &c->g_part[ph] : &c->g_cycle, g, nullptr,
+                              nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    if (e != hipSuccess)
+      return set_error(c, QDEQ_ERR_HIP,
+                       std::string("V-cycle graph capture: ") + hipGetErrorString(e));
+  }
+  c->g_timed = split;
   c->g_cycle_mode = mode;
   return QDEQ_OK;
 }
@@ -428,10 +458,25 @@ int mg_solve(qdeq_ctx *c, int *vcycles, double *res0, double *res) {
                          ")");
       break;
     }
-    QDEQ_HIP(c, hipGraphLaunch(c->g_cycle, c->stream));
+    if (c->g_timed) {
+      QDEQ_HIP(c, hipGraphLaunch(c->g_part[PH_PRE], c->stream));
+      QDEQ_HIP(c, hipEventRecord(c->ev_t0, c->stream));
+      QDEQ_HIP(c, hipGraphLaunch(c->g_part[PH_TIMED], c->stream));
+      QDEQ_HIP(c, hipEventRecord(c->ev_t1, c->stream));
+      QDEQ_HIP(c, hipGraphLaunch(c->g_part[PH_POST], c->stream));
+    } else {
+      QDEQ_HIP(c, hipGraphLaunch(c->g_cycle, c->stream));
+    }
     QDEQ_HIP(c, hipMemcpyAsync(c->scal_host, c->scal, sizeof(double),
                                hipMemcpyDeviceToHost, c->stream));
     QDEQ_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->g_timed) {
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1) == hipSuccess) {
+        c->timed_ms += ms;
+        c->timed_calls += 1;
+      }
+    }
     delta = c->scal_host[SC_NORM];
     it++;
     if (!std::isfinite(delta)) {

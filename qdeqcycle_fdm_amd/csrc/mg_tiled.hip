@@ -22,7 +22,9 @@
 
 namespace qdeq {
 
-constexpr int TILE_THREADS = 256;
+constexpr int TX_THREADS = 64;   // one wavefront across a tile row
+constexpr int TY_THREADS = 16;   // 16 rows of lanes: 1024 threads per tile
+constexpr int TILE_THREADS = TX_THREADS * TY_THREADS;
 constexpr int TAIL_THREADS = 1024;
 
 template <typename T, bool EXACT>
@@ -44,55 +46,68 @@ struct Tile {
   int R, C, LS, j0, k0;
 };
 
+// Loads the tile; columns are visited in unrolled groups of 64 lanes so the
+// global loads of a row are all in flight before the LDS stores.
 template <typename T, bool HAS_F>
 __device__ inline void tile_load(Tile<T> &t, const T *__restrict__ u,
                                  const T *__restrict__ f,
                                  const T *__restrict__ cx,
                                  const T *__restrict__ cz, int N, int K, int P,
                                  bool zero_u) {
-  const int total = t.R * t.C;
-  for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-    const int lr = idx / t.C, lc = idx - lr * t.C;
-    const int j = t.j0 + lr, k = t.k0 + lc;
-    const int s = lr * t.LS + lc;
-    const bool in = (j >= 0) && (j < N) && (k >= 0) && (k < K);
-    const size_t g = (size_t)j * P + k;
-    t.su[s] = (in && !zero_u) ? u[g] : T(0);
-    if (HAS_F) t.sf[s] = in ? f[g] : T(0);
-    t.scx[s] = in ? cx[g] : T(0);
-    t.scz[s] = (j >= 0 && j <= N && k >= 0 && k < K) ? cz[g] : T(0);
-  }
-  for (int lc = threadIdx.x; lc < t.C; lc += blockDim.x) {
-    const int j = t.j0 + t.R, k = t.k0 + lc;
-    const bool in = (j >= 0) && (j <= N) && (k >= 0) && (k < K);
-    t.scz[t.R * t.LS + lc] = in ? cz[(size_t)j * P + k] : T(0);
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  for (int lr = ty; lr <= t.R; lr += TY_THREADS) {
+    const int j = t.j0 + lr;
+    const bool jin = (j >= 0) && (j < N) && lr < t.R;
+    const bool jinz = (j >= 0) && (j <= N);
+    const size_t grow = (size_t)j * P;
+#pragma unroll
+    for (int cc = 0; cc < 5; cc++) {
+      const int lc = tx + TX_THREADS * cc;
+      if (lc >= t.C) break;
+      const int k = t.k0 + lc;
+      const bool kin = (k >= 0) && (k < K);
+      const int s = lr * t.LS + lc;
+      const bool in = jin && kin;
+      T vu = T(0), vf = T(0), vx = T(0), vz = T(0);
+      if (in && !zero_u) vu = u[grow + k];
+      if (HAS_F && in) vf = f[grow + k];
+      if (in) vx = cx[grow + k];
+      if (jinz && kin) vz = cz[grow + k];
+      if (lr < t.R) {
+        t.su[s] = vu;
+        if (HAS_F) t.sf[s] = vf;
+        t.scx[s] = vx;
+      }
+      t.scz[s] = vz;
+    }
   }
 }
 
-// Half-sweep `colour` over the local window [h, R-h) x [h, C-h).
+// Half-sweep `colour` over the local window [h, R-h) x [h, C-h); lanes run
+// over the points of one colour only.
 template <typename T, bool HAS_F, bool EXACT>
 __device__ inline void tile_half_sweep(Tile<T> &t, int h, int colour, int N,
                                        int K, T DH2, T relax) {
-  const int rows = t.R - 2 * h, cols = t.C - 2 * h;
-  if (rows <= 0 || cols <= 0) return;
+  const int cols = t.C - 2 * h;
   const int half = (cols + 1) / 2;
-  const int total = rows * half;
-  for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-    const int rr = idx / half, q = idx - rr * half;
-    const int lr = h + rr;
+  for (int lr = h + threadIdx.y; lr < t.R - h; lr += TY_THREADS) {
     const int j = t.j0 + lr;
-    const int lc = h + 2 * q + ((colour + j + t.k0 + h) & 1);
-    if (lc >= t.C - h) continue;
-    const int k = t.k0 + lc;
-    if (j < 0 || j >= N || k < 1 || k > K - 2) continue;
+    if (j < 0 || j >= N) continue;
+    const int delta = (colour + j + t.k0 + h) & 1;
     const int lS = (j == 0) ? lr + 1 : lr - 1;
     const int lN = (j == N - 1) ? lr - 1 : lr + 1;
-    const int s = lr * t.LS + lc;
-    const T m1 = t.scx[s], m2 = t.scx[s + 1], m3 = t.scz[s], m4 = t.scz[s + t.LS];
-    const T fv = HAS_F ? DH2 * t.sf[s] : T(0);
-    t.su[s] = rbgs_update<T, EXACT>(m1, m2, m3, m4, t.su[s - 1], t.su[s + 1],
-                                    t.su[lS * t.LS + lc], t.su[lN * t.LS + lc],
-                                    t.su[s], fv, HAS_F, relax);
+    for (int q = threadIdx.x; q < half; q += TX_THREADS) {
+      const int lc = h + 2 * q + delta;
+      if (lc >= t.C - h) continue;
+      const int k = t.k0 + lc;
+      if (k < 1 || k > K - 2) continue;
+      const int s = lr * t.LS + lc;
+      const T m1 = t.scx[s], m2 = t.scx[s + 1], m3 = t.scz[s], m4 = t.scz[s + t.LS];
+      const T fv = HAS_F ? DH2 * t.sf[s] : T(0);
+      t.su[s] = rbgs_update<T, EXACT>(m1, m2, m3, m4, t.su[s - 1], t.su[s + 1],
+                                      t.su[lS * t.LS + lc], t.su[lN * t.LS + lc],
+                                      t.su[s], fv, HAS_F, relax);
+    }
   }
 }
 
@@ -151,47 +166,50 @@ __global__ __launch_bounds__(TILE_THREADS) void k_down(
   tile_load<T, HAS_F>(t, uin, f, cx, cz, N, K, P, zero_u != 0);
   __syncthreads();
   tile_sweeps<T, HAS_F, EXACT>(t, nu, N, K, DH2, relax);
-  // residual on the interior plus one ring, kept in sf's place when HAS_F,
-  // or in the scratch row block after scz otherwise
+  // residual on the interior plus one ring; it replaces f (read at the same
+  // point only) or goes to the scratch block after scz
   T *sr = HAS_F ? t.sf : t.scz + (t.R + 1) * t.LS;
   const int h = H - 1;
-  {
-    const int rows = t.R - 2 * h, cols = t.C - 2 * h;
-    for (int idx = threadIdx.x; idx < rows * cols; idx += blockDim.x) {
-      const int lr = h + idx / cols, lc = h + idx % cols;
-      const int j = t.j0 + lr, k = t.k0 + lc;
+  for (int lr = h + threadIdx.y; lr < t.R - h; lr += TY_THREADS) {
+    const int j = t.j0 + lr;
+    for (int lc = h + threadIdx.x; lc < t.C - h; lc += TX_THREADS) {
+      const int k = t.k0 + lc;
       T r = T(0);
       if (j >= 0 && j < N && k >= 1 && k <= K - 2)
         r = tile_residual<T, HAS_F>(t, lr, lc, j, N, DH2);
-      // the residual overwrites f only after every thread has read f at this
-      // point: f is read at the point itself alone
       sr[lr * t.LS + lc] = r;
     }
   }
   __syncthreads();
   // restriction for the coarse points inside the tile interior
-  const int ncr = TY / 2, ncc = TX / 2;
   const int jc0 = (t.j0 + H) / 2, kc0 = (t.k0 + H) / 2;
-  for (int idx = threadIdx.x; idx < ncr * ncc; idx += blockDim.x) {
-    const int jc = jc0 + idx / ncc, kc = kc0 + idx % ncc;
-    if (jc >= Nc || kc < 1 || kc > Kc - 2) continue;
+  for (int a = threadIdx.y; a < TY / 2; a += TY_THREADS) {
+    const int jc = jc0 + a;
+    if (jc >= Nc) continue;
     const int jS = (jc == 0) ? 1 : 2 * jc - 1;
     const int jN = (jc == Nc - 1) ? 2 * jc - 1 : 2 * jc + 1;
     const T *rc = sr + (2 * jc - t.j0) * t.LS - t.k0;
     const T *rs = sr + (jS - t.j0) * t.LS - t.k0;
     const T *rn = sr + (jN - t.j0) * t.LS - t.k0;
-    const int kf = 2 * kc;
-    const T v = T(0.0625) *
-                (T(4.0) * rc[kf] + T(2.0) * rc[kf - 1] + T(2.0) * rc[kf + 1] +
-                 T(2.0) * rs[kf] + T(2.0) * rn[kf] + rs[kf - 1] + rs[kf + 1] +
-                 rn[kf - 1] + rn[kf + 1]);
-    r2[(size_t)jc * Pc + kc] = v;
+    for (int b = threadIdx.x; b < TX / 2; b += TX_THREADS) {
+      const int kc = kc0 + b;
+      if (kc < 1 || kc > Kc - 2) continue;
+      const int kf = 2 * kc;
+      const T v = T(0.0625) *
+                  (T(4.0) * rc[kf] + T(2.0) * rc[kf - 1] + T(2.0) * rc[kf + 1] +
+                   T(2.0) * rs[kf] + T(2.0) * rn[kf] + rs[kf - 1] + rs[kf + 1] +
+                   rn[kf - 1] + rn[kf + 1]);
+      r2[(size_t)jc * Pc + kc] = v;
+    }
   }
   // smoothed interior
-  for (int idx = threadIdx.x; idx < TY * TX; idx += blockDim.x) {
-    const int lr = H + idx / TX, lc = H + idx % TX;
-    const int j = t.j0 + lr, k = t.k0 + lc;
-    if (j < N && k < K) uout[(size_t)j * P + k] = t.su[lr * t.LS + lc];
+  for (int lr = H + threadIdx.y; lr < H + TY; lr += TY_THREADS) {
+    const int j = t.j0 + lr;
+    if (j >= N) continue;
+    for (int lc = H + threadIdx.x; lc < H + TX; lc += TX_THREADS) {
+      const int k = t.k0 + lc;
+      if (k < K) uout[(size_t)j * P + k] = t.su[lr * t.LS + lc];
+    }
   }
 }
 
@@ -199,10 +217,11 @@ __global__ __launch_bounds__(TILE_THREADS) void k_down(
 enum { UP_STORE = 0, UP_NORM = 1, UP_ADD64 = 2 };
 
 __device__ inline double block_sum(double v, double *sh) {
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  const int n = blockDim.x * blockDim.y;
   sh[tid] = v;
   __syncthreads();
-  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+  for (int s = n / 2; s > 0; s >>= 1) {
     if (tid < s) sh[tid] += sh[tid + s];
     __syncthreads();
   }
@@ -217,60 +236,68 @@ __global__ __launch_bounds__(TILE_THREADS) void k_up(
     T DH2, T relax, int nu, const T *__restrict__ e, int Nc, int Kc, int Pc,
     int TX, int TY, double *__restrict__ partial, double *__restrict__ u64) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ double red[TILE_THREADS];
   const int H = 2 * nu + (MODE == UP_NORM ? 1 : 0);
   Tile<T> t = make_tile<T>(smem, TX, TY, H, HAS_F);
   tile_load<T, HAS_F>(t, uin, f, cx, cz, N, K, P, false);
   __syncthreads();
   // u += P e (prolongate_kernel, odeSolver.cpp:1017-1043), point by point
-  for (int idx = threadIdx.x; idx < t.R * t.C; idx += blockDim.x) {
-    const int lr = idx / t.C, lc = idx - lr * t.C;
-    const int j = t.j0 + lr, k = t.k0 + lc;
-    if (j < 0 || j >= N || k < 0 || k >= K) continue;
-    const int cj = j >> 1, ck = k >> 1, dj = j & 1, dk = k & 1;
-    if (ck >= Kc - 1) continue;
+  for (int lr = threadIdx.y; lr < t.R; lr += TY_THREADS) {
+    const int j = t.j0 + lr;
+    if (j < 0 || j >= N) continue;
+    const int cj = j >> 1, dj = j & 1;
     if (!(cj < Nc - 1 || (FULL_TOP && cj == Nc - 1 && dj == 0))) continue;
-    const T *ec = e + (size_t)cj * Pc + ck;
-    const T e00 = ec[0];
-    T add;
-    if (!dj && !dk) {
-      add = e00;
-    } else if (!dj) {
-      add = T(0.5) * (e00 + ec[1]);
-    } else if (!dk) {
-      add = T(0.5) * (e00 + ec[Pc]);
-    } else {
-      add = T(0.25) * (e00 + ec[1] + ec[Pc] + ec[Pc + 1]);
+    const T *erow = e + (size_t)cj * Pc;
+    for (int lc = threadIdx.x; lc < t.C; lc += TX_THREADS) {
+      const int k = t.k0 + lc;
+      if (k < 0 || k >= K) continue;
+      const int ck = k >> 1, dk = k & 1;
+      if (ck >= Kc - 1) continue;
+      const T *ec = erow + ck;
+      const T e00 = ec[0];
+      T add;
+      if (!dj && !dk) {
+        add = e00;
+      } else if (!dj) {
+        add = T(0.5) * (e00 + ec[1]);
+      } else if (!dk) {
+        add = T(0.5) * (e00 + ec[Pc]);
+      } else {
+        add = T(0.25) * (e00 + ec[1] + ec[Pc] + ec[Pc + 1]);
+      }
+      t.su[lr * t.LS + lc] += add;
     }
-    const int s = lr * t.LS + lc;
-    t.su[s] += add;
   }
   __syncthreads();
   tile_sweeps<T, HAS_F, EXACT>(t, nu, N, K, DH2, relax);
   double acc = 0.0;
-  for (int idx = threadIdx.x; idx < TY * TX; idx += blockDim.x) {
-    const int lr = H + idx / TX, lc = H + idx % TX;
-    const int j = t.j0 + lr, k = t.k0 + lc;
-    if (j >= N || k >= K) continue;
-    const size_t g = (size_t)j * P + k;
-    if (MODE == UP_NORM && k >= 1 && k <= K - 2) {
-      const double r = (double)tile_residual<T, HAS_F>(t, lr, lc, j, N, DH2);
-      acc += r * r;
-    }
-    if (MODE == UP_ADD64) {
-      if (k >= 1 && k <= K - 2) u64[g] += (double)t.su[lr * t.LS + lc];
-    } else {
-      uout[g] = t.su[lr * t.LS + lc];
+  for (int lr = H + threadIdx.y; lr < H + TY; lr += TY_THREADS) {
+    const int j = t.j0 + lr;
+    if (j >= N) continue;
+    for (int lc = H + threadIdx.x; lc < H + TX; lc += TX_THREADS) {
+      const int k = t.k0 + lc;
+      if (k >= K) continue;
+      const size_t g = (size_t)j * P + k;
+      if (MODE == UP_NORM && k >= 1 && k <= K - 2) {
+        const double r = (double)tile_residual<T, HAS_F>(t, lr, lc, j, N, DH2);
+        acc += r * r;
+      }
+      if (MODE == UP_ADD64) {
+        if (k >= 1 && k <= K - 2) u64[g] += (double)t.su[lr * t.LS + lc];
+      } else {
+        uout[g] = t.su[lr * t.LS + lc];
+      }
     }
   }
   if (MODE == UP_NORM) {
+    __shared__ double red[TILE_THREADS];
     const double s = block_sum(acc, red);
-    if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = s;
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+      partial[blockIdx.y * gridDim.x + blockIdx.x] = s;
   }
 }
 
 // ---------------------------------------------------------------------------
-// Coarse tail: one workgroup, every level in LDS.
+// Coarse tail: one workgroup, every level (values and coefficients) in LDS.
 // ---------------------------------------------------------------------------
 
 struct TailLevel {
@@ -286,28 +313,46 @@ struct TailArgs {
   void *e_out;          // correction of the entry level (global)
   int nu_pre, nu_post, nu_coarse;
   double relax;
+  const void *minv;     // exact coarsest solve (column-major inverse) or null
+  int cn;               // coarsest unknowns for minv
 };
 
+template <typename T>
+struct TailBufs {
+  T *u, *f, *cx, *cz;  // compact, stride K; cz has N+1 rows
+};
+
+// Sweeps on one tail level.  With one_wave, the first wavefront alone runs
+// them (small coarsest grids), ordering its own LDS traffic between
+// half-sweeps with wavefront fences instead of workgroup barriers.
 template <typename T, bool EXACT>
-__device__ inline void tail_smooth(T *u, const T *f, const T *__restrict__ cx,
-                                   const T *__restrict__ cz, int N, int K,
-                                   int P, T DH2, T relax, int sweeps) {
+__device__ inline void tail_smooth(const TailBufs<T> &b, int N, int K, T DH2,
+                                   T relax, int sweeps, bool one_wave) {
   const int half = (K + 1) / 2;
+  const int tid = threadIdx.x;
+  const int nthr = one_wave ? 64 : (int)blockDim.x;
+  if (one_wave && tid >= 64) return;
   for (int s = 0; s < 2 * sweeps; s++) {
     const int colour = s & 1;
-    for (int idx = threadIdx.x; idx < N * half; idx += blockDim.x) {
+    for (int idx = tid; idx < N * half; idx += nthr) {
       const int j = idx / half, q = idx - j * half;
       const int k = 2 * q + ((j + colour) & 1);
       if (k < 1 || k > K - 2) continue;
       const int jS = (j == 0) ? 1 : j - 1;
       const int jN = (j == N - 1) ? N - 2 : j + 1;
-      const size_t g = (size_t)j * P + k;
       const int c = j * K + k;
-      u[c] = rbgs_update<T, EXACT>(cx[g], cx[g + 1], cz[g], cz[g + P], u[c - 1],
-                                   u[c + 1], u[jS * K + k], u[jN * K + k], u[c],
-                                   DH2 * f[c], true, relax);
+      b.u[c] = rbgs_update<T, EXACT>(b.cx[c], b.cx[c + 1], b.cz[c], b.cz[c + K],
+                                     b.u[c - 1], b.u[c + 1], b.u[jS * K + k],
+                                     b.u[jN * K + k], b.u[c], DH2 * b.f[c], true,
+                                     relax);
     }
-    __syncthreads();
+    if (one_wave) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+      __syncthreads();
+    }
   }
 }
 
@@ -315,57 +360,86 @@ template <typename T, bool EXACT, bool FULL_TOP>
 __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   T *base = reinterpret_cast<T *>(smem);
-  T *U[6], *F[6];
+  TailBufs<T> B[6];
   size_t off = 0;
   for (int l = 0; l < a.nlev; l++) {
     const int n = a.lv[l].N * a.lv[l].K;
-    U[l] = base + off;
-    F[l] = base + off + n;
-    off += 2 * (size_t)n;
+    B[l].u = base + off;
+    B[l].f = base + off + n;
+    B[l].cx = base + off + 2 * n;
+    B[l].cz = base + off + 3 * n;
+    off += 4 * (size_t)n + a.lv[l].K;
   }
   T *Rs = base + off;  // residual scratch (entry level size)
+  const int tid = threadIdx.x;
+  // coefficients of every tail level, and the entry level's right-hand side
+  for (int l = 0; l < a.nlev; l++) {
+    const TailLevel &L = a.lv[l];
+    const T *cx = reinterpret_cast<const T *>(L.cx);
+    const T *cz = reinterpret_cast<const T *>(L.cz);
+    for (int idx = tid; idx < (L.N + 1) * L.K; idx += blockDim.x) {
+      const int j = idx / L.K, k = idx - j * L.K;
+      B[l].cz[idx] = cz[(size_t)j * L.P + k];
+      if (j < L.N) B[l].cx[idx] = cx[(size_t)j * L.P + k];
+    }
+  }
   {
     const TailLevel &L0 = a.lv[0];
     const T *fin = reinterpret_cast<const T *>(a.f_in);
-    for (int idx = threadIdx.x; idx < L0.N * L0.K; idx += blockDim.x) {
+    for (int idx = tid; idx < L0.N * L0.K; idx += blockDim.x) {
       const int j = idx / L0.K, k = idx - j * L0.K;
-      U[0][idx] = T(0);
-      F[0][idx] = (k >= 1 && k <= L0.K - 2) ? fin[(size_t)j * L0.P + k] : T(0);
+      B[0].u[idx] = T(0);
+      B[0].f[idx] = (k >= 1 && k <= L0.K - 2) ? fin[(size_t)j * L0.P + k] : T(0);
     }
   }
   __syncthreads();
   const T relax = (T)a.relax;
   for (int l = 0; l < a.nlev; l++) {
     const TailLevel &L = a.lv[l];
-    const T *cx = reinterpret_cast<const T *>(L.cx);
-    const T *cz = reinterpret_cast<const T *>(L.cz);
     const T DH2 = (T)L.DH2;
     if (l == a.nlev - 1) {
-      tail_smooth<T, EXACT>(U[l], F[l], cx, cz, L.N, L.K, L.P, DH2, relax,
-                            a.nu_coarse);
+      if (a.minv) {
+        // e = B^{-1} (DH^2 f), rows summed in column order
+        const T *minv = reinterpret_cast<const T *>(a.minv);
+        const int W = L.K - 2;
+        for (int i = tid; i < a.cn; i += blockDim.x) {
+          const int j = i / W, k = i - j * W + 1;
+          Rs[i] = DH2 * B[l].f[j * L.K + k];
+        }
+        __syncthreads();
+        for (int row = tid; row < a.cn; row += blockDim.x) {
+          T acc = T(0);
+          for (int col = 0; col < a.cn; col++)
+            acc = acc + minv[(size_t)col * a.cn + row] * Rs[col];
+          const int j = row / W, k = row - j * W + 1;
+          B[l].u[j * L.K + k] = acc;
+        }
+      } else {
+        const bool one_wave = L.N * L.K <= 512;
+        tail_smooth<T, EXACT>(B[l], L.N, L.K, DH2, relax, a.nu_coarse, one_wave);
+      }
+      __syncthreads();
       break;
     }
-    tail_smooth<T, EXACT>(U[l], F[l], cx, cz, L.N, L.K, L.P, DH2, relax, a.nu_pre);
-    // residual
-    for (int idx = threadIdx.x; idx < L.N * L.K; idx += blockDim.x) {
+    tail_smooth<T, EXACT>(B[l], L.N, L.K, DH2, relax, a.nu_pre, false);
+    for (int idx = tid; idx < L.N * L.K; idx += blockDim.x) {
       const int j = idx / L.K, k = idx - j * L.K;
       T r = T(0);
       if (k >= 1 && k <= L.K - 2) {
         const int jS = (j == 0) ? 1 : j - 1;
         const int jN = (j == L.N - 1) ? L.N - 2 : j + 1;
-        const size_t g = (size_t)j * L.P + k;
-        const T m1 = cx[g], m2 = cx[g + 1], m3 = cz[g], m4 = cz[g + L.P];
-        const T *u = U[l];
-        r = F[l][idx] - (m1 * u[idx - 1] + m2 * u[idx + 1] + m3 * u[jS * L.K + k] +
-                         m4 * u[jN * L.K + k] - (m1 + m2 + m3 + m4) * u[idx]) /
-                            DH2;
+        const T m1 = B[l].cx[idx], m2 = B[l].cx[idx + 1], m3 = B[l].cz[idx],
+                m4 = B[l].cz[idx + L.K];
+        const T *u = B[l].u;
+        r = B[l].f[idx] - (m1 * u[idx - 1] + m2 * u[idx + 1] + m3 * u[jS * L.K + k] +
+                           m4 * u[jN * L.K + k] - (m1 + m2 + m3 + m4) * u[idx]) /
+                              DH2;
       }
       Rs[idx] = r;
     }
     __syncthreads();
-    // restriction to level l+1, zero correction
     const TailLevel &C = a.lv[l + 1];
-    for (int idx = threadIdx.x; idx < C.N * C.K; idx += blockDim.x) {
+    for (int idx = tid; idx < C.N * C.K; idx += blockDim.x) {
       const int j = idx / C.K, k = idx - j * C.K;
       T v = T(0);
       if (k >= 1 && k <= C.K - 2) {
@@ -380,20 +454,19 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailArgs a) {
              T(2.0) * rs[kf] + T(2.0) * rn[kf] + rs[kf - 1] + rs[kf + 1] +
              rn[kf - 1] + rn[kf + 1]);
       }
-      F[l + 1][idx] = v;
-      U[l + 1][idx] = T(0);
+      B[l + 1].f[idx] = v;
+      B[l + 1].u[idx] = T(0);
     }
     __syncthreads();
   }
   for (int l = a.nlev - 2; l >= 0; l--) {
     const TailLevel &L = a.lv[l];
     const TailLevel &C = a.lv[l + 1];
-    // prolongation: one thread per coarse cell (disjoint fine points)
     const int jmax = FULL_TOP ? C.N : C.N - 1;
-    for (int idx = threadIdx.x; idx < jmax * (C.K - 1); idx += blockDim.x) {
+    for (int idx = tid; idx < jmax * (C.K - 1); idx += blockDim.x) {
       const int j = idx / (C.K - 1), k = idx - j * (C.K - 1);
-      const T *ec = U[l + 1] + j * C.K + k;
-      T *uf = U[l] + (2 * j) * L.K + 2 * k;
+      const T *ec = B[l + 1].u + j * C.K + k;
+      T *uf = B[l].u + (2 * j) * L.K + 2 * k;
       const T e00 = ec[0], e01 = ec[1];
       uf[0] += e00;
       uf[1] += T(0.5) * (e00 + e01);
@@ -405,16 +478,14 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailArgs a) {
       }
     }
     __syncthreads();
-    tail_smooth<T, EXACT>(U[l], F[l], reinterpret_cast<const T *>(L.cx),
-                          reinterpret_cast<const T *>(L.cz), L.N, L.K, L.P,
-                          (T)L.DH2, relax, a.nu_post);
+    tail_smooth<T, EXACT>(B[l], L.N, L.K, (T)L.DH2, relax, a.nu_post, false);
   }
   {
     const TailLevel &L0 = a.lv[0];
     T *eo = reinterpret_cast<T *>(a.e_out);
-    for (int idx = threadIdx.x; idx < L0.N * L0.K; idx += blockDim.x) {
+    for (int idx = tid; idx < L0.N * L0.K; idx += blockDim.x) {
       const int j = idx / L0.K, k = idx - j * L0.K;
-      eo[(size_t)j * L0.P + k] = U[0][idx];
+      eo[(size_t)j * L0.P + k] = B[0].u[idx];
     }
   }
 }
@@ -423,7 +494,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailArgs a) {
 // Host side
 // ---------------------------------------------------------------------------
 
-constexpr int TAIL_MAX_N = 65;
+constexpr int TAIL_MAX_N = 33;
 
 static size_t tile_lds_bytes(int TX, int TY, int H, bool has_f, size_t esz,
                              bool down) {
@@ -446,8 +517,10 @@ int tail_entry_level(const qdeq_ctx *c) {
 template <typename T>
 static size_t tail_lds_bytes(const qdeq_ctx *c, int lt) {
   size_t n = 0;
-  for (int l = lt; l < c->L; l++) n += 2 * (size_t)c->lv[l].N * c->lv[l].K;
-  n += (size_t)c->lv[lt].N * c->lv[lt].K;
+  for (int l = lt; l < c->L; l++)
+    n += 4 * (size_t)c->lv[l].N * c->lv[l].K + c->lv[l].K;
+  const size_t nr = (size_t)c->lv[lt].N * c->lv[lt].K;
+  n += nr > (size_t)c->coarse_n ? nr : (size_t)c->coarse_n;
   return n * sizeof(T);
 }
 
@@ -471,12 +544,16 @@ bool tiles_fit(const qdeq_ctx *c, int tx, int ty) {
   const size_t esz = fp64 ? 8 : 4;
   const size_t down = tile_lds_bytes(tx, ty, 2 * c->cfg.n_pre + 2, true, esz, true);
   const size_t up = tile_lds_bytes(tx, ty, 2 * c->cfg.n_post + 1, true, esz, false);
-  return down + 4096 <= c->lds_max && up + 4096 <= c->lds_max;
+  // tile_load covers at most 5 x 64 columns per row
+  if (tx + 2 * (2 * c->cfg.n_pre + 2) > 5 * TX_THREADS) return false;
+  if (tx + 2 * (2 * c->cfg.n_post + 1) > 5 * TX_THREADS) return false;
+  return down + 8192 <= c->lds_max && up + 8192 <= c->lds_max;
 }
 
 template <typename T, bool EXACT>
 static void launch_down(qdeq_ctx *c, LevelBufs<T> *B, int l, bool has_f,
                         bool zero_u) {
+  if (!emit_now(c)) return;
   const LevelDims &d = c->lv[l];
   const LevelDims &dc = c->lv[l + 1];
   int TX, TY;
@@ -487,12 +564,12 @@ static void launch_down(qdeq_ctx *c, LevelBufs<T> *B, int l, bool has_f,
   const size_t lds = tile_lds_bytes(TX, TY, H, has_f, sizeof(T), true);
   const T DH2 = (T)(d.DH * d.DH), relax = (T)c->cfg.relax;
   if (has_f)
-    hipLaunchKernelGGL((k_down<T, true, EXACT>), grd, dim3(TILE_THREADS), lds,
+    hipLaunchKernelGGL((k_down<T, true, EXACT>), grd, dim3(TX_THREADS, TY_THREADS), lds,
                        c->stream, B[l].u, B[l].r, B[l].f, B[l].cx, B[l].cz, d.N,
                        d.K, d.P, DH2, relax, nu, zero_u ? 1 : 0, B[l + 1].f,
                        dc.N, dc.K, dc.P, TX, TY);
   else
-    hipLaunchKernelGGL((k_down<T, false, EXACT>), grd, dim3(TILE_THREADS), lds,
+    hipLaunchKernelGGL((k_down<T, false, EXACT>), grd, dim3(TX_THREADS, TY_THREADS), lds,
                        c->stream, B[l].u, B[l].r, B[l].f, B[l].cx, B[l].cz, d.N,
                        d.K, d.P, DH2, relax, nu, zero_u ? 1 : 0, B[l + 1].f,
                        dc.N, dc.K, dc.P, TX, TY);
@@ -501,6 +578,11 @@ static void launch_down(qdeq_ctx *c, LevelBufs<T> *B, int l, bool has_f,
 template <typename T, bool EXACT, bool HAS_F, int MODE>
 static void launch_up_t(qdeq_ctx *c, LevelBufs<T> *B, int l, double *partial,
                         double *u64, int *nblocks) {
+  if (nblocks) {
+    const LevelDims &d0 = c->lv[l];
+    *nblocks = ((d0.K + c->tile_tx - 1) / c->tile_tx) * ((d0.N + c->tile_ty - 1) / c->tile_ty);
+  }
+  if (!emit_now(c)) return;
   const LevelDims &d = c->lv[l];
   const LevelDims &dc = c->lv[l + 1];
   int TX, TY;
@@ -511,7 +593,7 @@ static void launch_up_t(qdeq_ctx *c, LevelBufs<T> *B, int l, double *partial,
   if (nblocks) *nblocks = (int)(grd.x * grd.y);
   const size_t lds = tile_lds_bytes(TX, TY, H, HAS_F, sizeof(T), false);
   const T DH2 = (T)(d.DH * d.DH), relax = (T)c->cfg.relax;
-  hipLaunchKernelGGL((k_up<T, HAS_F, EXACT, true, MODE>), grd, dim3(TILE_THREADS),
+  hipLaunchKernelGGL((k_up<T, HAS_F, EXACT, true, MODE>), grd, dim3(TX_THREADS, TY_THREADS),
                      lds, c->stream, B[l].r, B[l].u, B[l].f, B[l].cx, B[l].cz,
                      d.N, d.K, d.P, DH2, relax, nu, B[l + 1].u, dc.N, dc.K, dc.P,
                      TX, TY, partial, u64);
@@ -519,6 +601,7 @@ static void launch_up_t(qdeq_ctx *c, LevelBufs<T> *B, int l, double *partial,
 
 template <typename T, bool EXACT>
 static void launch_tail(qdeq_ctx *c, LevelBufs<T> *B, int lt) {
+  if (!emit_now(c)) return;
   TailArgs a{};
   a.nlev = c->L - lt;
   for (int i = 0; i < a.nlev; i++) {
@@ -531,6 +614,10 @@ static void launch_tail(qdeq_ctx *c, LevelBufs<T> *B, int lt) {
   a.nu_post = c->cfg.n_post;
   a.nu_coarse = c->cfg.n_coarse;
   a.relax = c->cfg.relax;
+  a.cn = c->coarse_n;
+  a.minv = c->coarse_n > 0 ? (sizeof(T) == 8 ? (const void *)c->cinv64
+                                             : (const void *)c->cinv32)
+                           : nullptr;
   hipLaunchKernelGGL((k_tail<T, EXACT, true>), dim3(1), dim3(TAIL_THREADS),
                      tail_lds_bytes<T>(c, lt), c->stream, a);
 }
@@ -544,17 +631,24 @@ template <typename T, bool EXACT>
 static void vcycle_tiled(qdeq_ctx *c, LevelBufs<T> *B, bool level0_has_f,
                          double *partial, double *u64, int *nparts) {
   const int lt = tail_entry_level(c);
-  for (int l = 0; l < lt; l++)
+  for (int l = 0; l < lt; l++) {
+    if (l == 1) timing_mark(c, QDEQ_TIME_DOWN1, 0);
     launch_down<T, EXACT>(c, B, l, l > 0 || level0_has_f, l > 0 || level0_has_f);
+    if (l == 1) timing_mark(c, QDEQ_TIME_DOWN1, 1);
+  }
+  timing_mark(c, QDEQ_TIME_TAIL, 0);
   launch_tail<T, EXACT>(c, B, lt);
+  timing_mark(c, QDEQ_TIME_TAIL, 1);
   for (int l = lt - 1; l >= 1; l--)
     launch_up_t<T, EXACT, true, UP_STORE>(c, B, l, nullptr, nullptr, nullptr);
+  timing_mark(c, QDEQ_TIME_UP0, 0);
   if (u64)
     launch_up_t<T, EXACT, true, UP_ADD64>(c, B, 0, nullptr, u64, nullptr);
   else if (level0_has_f)
     launch_up_t<T, EXACT, true, UP_STORE>(c, B, 0, nullptr, nullptr, nullptr);
   else
     launch_up_t<T, EXACT, false, UP_NORM>(c, B, 0, partial, nullptr, nparts);
+  timing_mark(c, QDEQ_TIME_UP0, 1);
 }
 
 void vcycle_tiled_fp64(qdeq_ctx *c, double *partial, int *nparts) {

@@ -85,6 +85,20 @@ struct qdeq_ctx {
   bool tiled = true;
   int tile_tx = 64, tile_ty = 32;
   size_t lds_max = 65536;  // dynamic LDS per workgroup (queried at creation)
+  // exact coarsest-level solve of the fast cycle (n_coarse == 0): inverse of
+  // the coarsest operator, column-major, FP64 and FP32 copies
+  double *cinv64 = nullptr;
+  float *cinv32 = nullptr;
+  int coarse_n = 0;
+  // HIP-event timing of one kernel of the fast cycle (0 = off; see
+  // qdeq_set_kernel_timing): events bracket it inside the captured graph
+  int timing = 0;
+  bool g_timed = false;
+  hipGraphExec_t g_part[4] = {};   // pre / timed kernel / post graphs
+  int emit_phase = 0, cur_phase = 1;
+  hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;
+  double timed_ms = 0.0;
+  long long timed_calls = 0;
 };
 
 namespace qdeq {
@@ -104,6 +118,23 @@ void vcycle_tiled_fp64(qdeq_ctx *c, double *partial, int *nparts);
 void vcycle_tiled_fp32_ir(qdeq_ctx *c);
 int tiled_setup(qdeq_ctx *c);
 bool tiles_fit(const qdeq_ctx *c, int tx, int ty);
+
+// Kernel timing splits the captured iteration into three graphs around the
+// timed kernel; launches are emitted only in their own phase when a part is
+// being captured (emit_phase), and every launch when it is 0.
+enum { PH_ALL = 0, PH_PRE = 1, PH_TIMED = 2, PH_POST = 3 };
+inline bool emit_now(const qdeq_ctx *c) {
+  return c->emit_phase == PH_ALL || c->emit_phase == c->cur_phase;
+}
+// marks the start (pos 0) or end (pos 1) of kernel `which` in the sequence
+inline void timing_mark(qdeq_ctx *c, int which, int pos) {
+  if (c->timing == which) c->cur_phase = pos ? PH_POST : PH_TIMED;
+}
+
+// exact coarse solve (coarse.hip)
+int build_coarse_inverse(qdeq_ctx *c);
+void launch_coarse_direct_fp64(qdeq_ctx *c, double *u, const double *f);
+void launch_coarse_direct_fp32(qdeq_ctx *c, float *u, const float *f);
 
 // medium (medium.hip)
 int build_mu_equi_device(int ni, int nk, const double *mu_dev, double *me_dev,

@@ -38,7 +38,9 @@ EXPORTS = (
     "qdeq_nr_search", "qdeq_cal_dpsi", "qdeq_compute_error",
     "qdeq_compute_step_size", "qdeq_rk43_setup", "qdeq_rk43_step",
     "qdeq_rk43_get_state", "qdeq_set_tiles", "qdeq_uses_tiled_cycle",
+    "qdeq_set_kernel_timing", "qdeq_get_kernel_timing", "qdeq_copy_probe",
 )
+TIME_OFF, TIME_UP0, TIME_TAIL, TIME_DOWN1 = 0, 1, 2, 3
 
 
 class QdeqError(RuntimeError):
@@ -118,6 +120,9 @@ def load(path: str = LIB_PATH):
     L.qdeq_compute_step_size.restype = ctypes.c_double
     L.qdeq_set_tiles.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.qdeq_uses_tiled_cycle.argtypes = [_vp]
+    L.qdeq_set_kernel_timing.argtypes = [_vp, ctypes.c_int]
+    L.qdeq_get_kernel_timing.argtypes = [_vp, _dp, ctypes.POINTER(ctypes.c_longlong)]
+    L.qdeq_copy_probe.argtypes = [_vp, ctypes.c_longlong, ctypes.c_int, _dp]
     L.qdeq_rk43_setup.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp]
     L.qdeq_rk43_step.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(StepInfo)]
     L.qdeq_rk43_get_state.argtypes = [_vp, ctypes.POINTER(StepInfo), _dp, _dp,
@@ -232,6 +237,23 @@ class Engine:
     @property
     def uses_tiled_cycle(self) -> bool:
         return bool(load().qdeq_uses_tiled_cycle(self.h))
+
+    def set_kernel_timing(self, which: int):
+        """HIP-event timing of one fast-cycle kernel (TIME_* selector)."""
+        self._check(load().qdeq_set_kernel_timing(self.h, which))
+
+    def kernel_timing(self):
+        """(total_ms, calls) accumulated since set_kernel_timing."""
+        ms = ctypes.c_double(0)
+        n = ctypes.c_longlong(0)
+        self._check(load().qdeq_get_kernel_timing(self.h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def copy_probe(self, n_doubles: int, reps: int = 20) -> float:
+        """Average ms of one plain device copy of n_doubles doubles."""
+        ms = ctypes.c_double(0)
+        self._check(load().qdeq_copy_probe(self.h, n_doubles, reps, ctypes.byref(ms)))
+        return ms.value
 
     def mu_equi(self) -> np.ndarray:
         out = np.zeros((2 * self.nz - 1, 2 * self.nx - 1))

@@ -15,7 +15,7 @@ from qdeqcycle_fdm_amd import config, lib
 
 pytestmark = pytest.mark.gpu
 
-FAST = dict(cycle=config.CYCLE_FAST, n_pre=2, n_post=2, n_coarse=32)
+FAST = dict(cycle=config.CYCLE_FAST, n_pre=2, n_post=2, n_coarse=0)
 
 
 def _perturbed(pb, seed, scale=0.5):
@@ -25,10 +25,12 @@ def _perturbed(pb, seed, scale=0.5):
     return U
 
 
-def _fast_oracle_solve(pb, me, U, ncap, tol=1e-300, mode=oracle.ORC_TOL_ABS):
+def _fast_oracle_solve(pb, me, U, ncap, tol=1e-300, mode=oracle.ORC_TOL_ABS,
+                      n_coarse=None):
     p = pb.params
     mg = oracle.MG(p.ni, p.nk, p.dh, p.n_levels, me)
-    mg.set_cycle(FAST["n_pre"], FAST["n_post"], FAST["n_coarse"], True)
+    mg.set_cycle(FAST["n_pre"], FAST["n_post"],
+                 FAST["n_coarse"] if n_coarse is None else n_coarse, True)
     V = U.copy()
     st, vc, r0, r = mg.solve(V, mode, tol, ncap)
     return V, vc, r0, r
@@ -50,16 +52,20 @@ def _gpu_solve(pb, me, U, tiled, tile=(64, 16), **kw):
 
 @pytest.mark.parametrize("n,levels", [(65, 3), (129, 4), (257, 5)])
 @pytest.mark.parametrize("ncap", [1, 3])
-def test_fast_fp64_tiled_equals_untiled_equals_oracle(n, levels, ncap):
+@pytest.mark.parametrize("n_coarse", [0, 32])
+def test_fast_fp64_tiled_equals_untiled_equals_oracle(n, levels, ncap, n_coarse):
+    """n_coarse = 0: exact coarsest solve; 32: coarsest sweeps."""
     pb = config.make_bp1_problem(n, basin=True, levels=levels)
     me = oracle.mu_equi(pb.mu)
     U = _perturbed(pb, 3)
-    ref, vc, _, _ = _fast_oracle_solve(pb, me, U, ncap)
+    ref, vc, _, _ = _fast_oracle_solve(pb, me, U, ncap, n_coarse=n_coarse)
     assert vc == ncap
-    untiled, _ = _gpu_solve(pb, me, U, False, tol=1e-300, max_vcycles=ncap)
+    untiled, _ = _gpu_solve(pb, me, U, False, tol=1e-300, max_vcycles=ncap,
+                            n_coarse=n_coarse)
     np.testing.assert_array_equal(untiled, ref)
     for tile in ((64, 16), (32, 8), (128, 4)):
-        tiled, _ = _gpu_solve(pb, me, U, True, tile=tile, tol=1e-300, max_vcycles=ncap)
+        tiled, _ = _gpu_solve(pb, me, U, True, tile=tile, tol=1e-300, max_vcycles=ncap,
+                              n_coarse=n_coarse)
         np.testing.assert_array_equal(tiled, ref, err_msg=str(tile))
 
 
