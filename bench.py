@@ -49,12 +49,11 @@ def log(*a):
 
 
 def sync():
-    try:
-        import torch
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:
-        pass
+    """Device-wide barrier through the engine's HIP runtime.  torch is not
+    imported here: its bundled HIP runtime shares the SONAME of /opt/rocm's,
+    would be the one the library binds to, and does not match rocprofv3."""
+    from qdeqcycle_fdm_amd import lib
+    lib.device_synchronize(0)
 
 
 def make_engine(pb, me, mode):
@@ -151,6 +150,13 @@ def main():
     ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-kernel-timing", action="store_true",
+                    help="run the V-cycle as one captured graph without the "
+                         "HIP-event bracket of the level-0 up kernel (used for "
+                         "the rocprofv3 passes)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="N",
+                    help="only time a plain device copy of N doubles (counter "
+                         "calibration) and exit")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -164,11 +170,18 @@ def main():
     lib.load()
     pb = config.make_bp1_problem(1025, basin=True, tol_mode=config.TOL_REL,
                                  tol=1e-10)
+    if args.copy_probe:
+        with lib.Engine(pb) as eng:
+            ms = eng.copy_probe(args.copy_probe, reps=10)
+        print(json.dumps({"copy_probe_doubles": args.copy_probe, "avg_ms": ms,
+                          "GB/s": 2 * 8 * args.copy_probe / ms / 1e6}))
+        return
     t0 = time.perf_counter()
     me = lib.build_mu_equi(pb.mu)
     log(f"mu_equi on the GPU: {time.perf_counter() - t0:.2f} s")
 
-    timed = lib.TIME_UP0 if args.mode != "ref64" else None
+    timed = (lib.TIME_UP0 if args.mode != "ref64" and not args.no_kernel_timing
+             else None)
     eng, main_run = run_mode(pb, me, args.mode, args.warmup, args.steps, timed)
     log(f"{args.mode}: {main_run['steps_per_s']:.2f} steps/s, "
         f"{main_run['vcycles_per_solve']:.2f} V-cycles/solve, "

@@ -116,6 +116,8 @@ int qdeq_uses_tiled_cycle(const qdeq_ctx *ctx);
 enum { QDEQ_TIME_OFF = 0, QDEQ_TIME_UP0 = 1, QDEQ_TIME_TAIL = 2,
        QDEQ_TIME_DOWN1 = 3 };
 int qdeq_set_kernel_timing(qdeq_ctx *ctx, int which);
+/* hipDeviceSynchronize on `device` (a barrier for host-side timing). */
+int qdeq_device_synchronize(int device);
 int qdeq_get_kernel_timing(qdeq_ctx *ctx, double *total_ms, long long *calls);
 int qdeq_copy_probe(qdeq_ctx *ctx, long long n_doubles, int reps, double *avg_ms);
 

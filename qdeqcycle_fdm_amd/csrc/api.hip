@@ -248,6 +248,13 @@ int qdeq_set_tiles(qdeq_ctx *c, int tiled, int tile_x, int tile_y) {
 
 int qdeq_uses_tiled_cycle(const qdeq_ctx *c) { return c && use_tiled(c) ? 1 : 0; }
 
+int qdeq_device_synchronize(int device) {
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) return set_error(nullptr, QDEQ_ERR_HIP, hipGetErrorString(e));
+  return QDEQ_OK;
+}
+
 int qdeq_set_kernel_timing(qdeq_ctx *c, int which) {
   if (!c || which < 0 || which > QDEQ_TIME_DOWN1)
     return set_error(c, QDEQ_ERR_ARG, "bad timing selector");

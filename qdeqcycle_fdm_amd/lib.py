@@ -39,6 +39,7 @@ EXPORTS = (
     "qdeq_compute_step_size", "qdeq_rk43_setup", "qdeq_rk43_step",
     "qdeq_rk43_get_state", "qdeq_set_tiles", "qdeq_uses_tiled_cycle",
     "qdeq_set_kernel_timing", "qdeq_get_kernel_timing", "qdeq_copy_probe",
+    "qdeq_device_synchronize",
 )
 TIME_OFF, TIME_UP0, TIME_TAIL, TIME_DOWN1 = 0, 1, 2, 3
 
@@ -123,6 +124,7 @@ def load(path: str = LIB_PATH):
     L.qdeq_set_kernel_timing.argtypes = [_vp, ctypes.c_int]
     L.qdeq_get_kernel_timing.argtypes = [_vp, _dp, ctypes.POINTER(ctypes.c_longlong)]
     L.qdeq_copy_probe.argtypes = [_vp, ctypes.c_longlong, ctypes.c_int, _dp]
+    L.qdeq_device_synchronize.argtypes = [ctypes.c_int]
     L.qdeq_rk43_setup.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp]
     L.qdeq_rk43_step.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(StepInfo)]
     L.qdeq_rk43_get_state.argtypes = [_vp, ctypes.POINTER(StepInfo), _dp, _dp,
@@ -169,6 +171,13 @@ def make_config(params: cfgmod.SolverParams, device: int = 0) -> QdeqConfig:
     c.f0, c.V0, c.L, c.Vini = params.f0, params.V0, params.L, params.Vini
     c.device = device
     return c
+
+
+def device_synchronize(device: int = 0) -> None:
+    """Waits for all work on the device (host-side timing barrier)."""
+    st = load().qdeq_device_synchronize(device)
+    if st != QDEQ_OK:
+        raise QdeqError(st, load().qdeq_global_error().decode())
 
 
 def build_mu_equi(mu: np.ndarray, device: int = 0) -> np.ndarray:
