@@ -91,6 +91,7 @@ static void free_ctx(qdeq_ctx *c) {
   if (c->flags_host) (void)hipHostFree(c->flags_host);
   if (c->ev_t0) (void)hipEventDestroy(c->ev_t0);
   if (c->ev_t1) (void)hipEventDestroy(c->ev_t1);
+  guess_free_ctx(c);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -528,8 +529,11 @@ static int check_flags(qdeq_ctx *c) {
 // One stage: boundary data, elliptic solve, fault update.
 static int run_stage(qdeq_ctx *c, int s) {
   launch_stage(c, s, c->t, c->dt);
-  int st = mg_solve(c, nullptr, nullptr, nullptr);
+  int st = guess_before_solve(c);
   if (st != QDEQ_OK) return st;
+  st = mg_solve(c, nullptr, nullptr, nullptr);
+  if (st != QDEQ_OK) return st;
+  if ((st = guess_after_solve(c)) != QDEQ_OK) return st;
   launch_fault_update(c, s);
   return QDEQ_OK;
 }
@@ -574,6 +578,7 @@ int qdeq_rk43_setup(qdeq_ctx *c, const double *a, const double *b,
   c->totErr = 0.0;
   c->vcycles = c->solves = c->ir_sweeps = 0;
   c->rk_ready = false;
+  guess_reset(c);
   return QDEQ_OK;
 }
 

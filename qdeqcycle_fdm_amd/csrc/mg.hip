@@ -345,6 +345,18 @@ static void launch_resnorm(qdeq_ctx *c, bool store32) {
                      c->partial, NORM_BLOCKS, c->scal + SC_NORM);
 }
 
+// Residual norm of an arbitrary level-0 displacement u (same operator and
+// summation as the stop test) into scal[slot]; partials at partial + off.
+void launch_resnorm_of(qdeq_ctx *c, const double *u, int off, int slot) {
+  const LevelDims &d = c->lv[0];
+  hipLaunchKernelGGL((k_resnorm_partial<false>), dim3(NORM_BLOCKS),
+                     dim3(NORM_THREADS), 0, c->stream, u, c->d[0].cx, c->d[0].cz,
+                     d.N, d.K, d.P, d.DH * d.DH, c->partial + off,
+                     (float *)nullptr, (float *)nullptr);
+  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(NORM_THREADS), 0, c->stream,
+                     c->partial + off, NORM_BLOCKS, c->scal + slot);
+}
+
 static void launch_bdres(qdeq_ctx *c) {
   const LevelDims &d = c->lv[0];
   const int nb = 8;
@@ -440,10 +452,29 @@ int mg_solve(qdeq_ctx *c, int *vcycles, double *res0, double *res) {
   if (c->cfg.tol_mode == QDEQ_TOL_REL) launch_bdres(c);
   launch_resnorm(c, ir);
   QDEQ_HIP(c, hipGetLastError());
-  QDEQ_HIP(c, hipMemcpyAsync(c->scal_host, c->scal, 2 * sizeof(double),
+  // a predicted start (learned guess) was enqueued before this solve: its
+  // candidate sits in the level-0 r buffer, its residual norm in
+  // scal[SC_NORM_PRED]
+  const bool cand = c->cand_pending;
+  c->cand_pending = false;
+  QDEQ_HIP(c, hipMemcpyAsync(c->scal_host, c->scal,
+                             (cand ? SC_NORM_PRED + 1 : 2) * sizeof(double),
                              hipMemcpyDeviceToHost, c->stream));
   QDEQ_HIP(c, hipStreamSynchronize(c->stream));
   double delta = c->scal_host[SC_NORM];
+  if (cand) {
+    // used only if its residual is finite and below the previous solution's
+    const double pn = c->scal_host[SC_NORM_PRED];
+    if (std::isfinite(pn) && pn < delta) {
+      const LevelDims &d = c->lv[0];
+      QDEQ_HIP(c, hipMemcpyAsync(c->d[0].u, c->d[0].r,
+                                 (size_t)d.N * d.P * sizeof(double),
+                                 hipMemcpyDeviceToDevice, c->stream));
+      if (ir) launch_resnorm(c, true);  // FP32 residual of the new start
+      delta = pn;
+      c->g_accepted++;
+    }
+  }
   const double r0 =
       c->cfg.tol_mode == QDEQ_TOL_REL ? c->scal_host[SC_NORM0] : delta;
   const double thr =

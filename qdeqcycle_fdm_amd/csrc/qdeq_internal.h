@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "qdeq.h"
+#include "qdeq_guess.h"
 
 #define QDEQ_MAX_LEVELS 12
 
@@ -50,6 +51,7 @@ enum {
   SC_TOTERR = 2,
   SC_VMAX = 3,
   SC_TSMAX = 4,
+  SC_NORM_PRED = 5,  // residual norm of the learned-guess candidate
   SC_COUNT = 8
 };
 
@@ -99,6 +101,20 @@ struct qdeq_ctx {
   hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;
   double timed_ms = 0.0;
   long long timed_calls = 0;
+  // learned initial guess (guess.hip): attached network, mode, the response
+  // W_R to unit remote data, inference scratch (16 padded rows) and the
+  // boundary data of the previous solve
+  qdeq_guess *guess = nullptr;
+  int guess_mode = 0;
+  double *wr = nullptr;
+  float *gx = nullptr, *gz = nullptr, *gy = nullptr, *gsplit = nullptr;
+  size_t gsplit_n = 0;
+  __bf16 *gh[4] = {};
+  double *gprev = nullptr, *gbefore = nullptr;
+  double *gs = nullptr;  // [0] max|ds|, [1] remote change, [2] test factor
+  bool gprev_valid = false, cand_pending = false, harvest_pending = false;
+  long long g_predicted = 0, g_accepted = 0, g_harvested = 0, g_seen = 0;
+  hipGraphExec_t g_pred = nullptr;
 };
 
 namespace qdeq {
@@ -110,6 +126,14 @@ int mg_solve(qdeq_ctx *c, int *vcycles, double *res0, double *res);
 int mg_build_coefficients(qdeq_ctx *c);
 void mg_destroy_graphs(qdeq_ctx *c);
 bool use_tiled(const qdeq_ctx *c);
+
+void launch_resnorm_of(qdeq_ctx *c, const double *u, int off, int slot);
+
+// learned initial guess (guess.hip)
+int guess_before_solve(qdeq_ctx *c);
+int guess_after_solve(qdeq_ctx *c);
+void guess_reset(qdeq_ctx *c);
+void guess_free_ctx(qdeq_ctx *c);
 
 // tiled fast cycle (mg_tiled.hip)
 int tail_entry_level(const qdeq_ctx *c);

@@ -40,8 +40,17 @@ EXPORTS = (
     "qdeq_rk43_get_state", "qdeq_set_tiles", "qdeq_uses_tiled_cycle",
     "qdeq_set_kernel_timing", "qdeq_get_kernel_timing", "qdeq_copy_probe",
     "qdeq_device_synchronize",
+    # include/qdeq_guess.h
+    "qdeq_guess_default_config", "qdeq_train_default_config",
+    "qdeq_guess_create", "qdeq_guess_destroy", "qdeq_guess_last_error",
+    "qdeq_guess_num_params", "qdeq_guess_dims", "qdeq_guess_get_params",
+    "qdeq_guess_set_params", "qdeq_guess_pairs", "qdeq_guess_clear_pairs",
+    "qdeq_guess_read_pairs", "qdeq_guess_train", "qdeq_guess_forward_backward",
+    "qdeq_guess_forward", "qdeq_attach_guess", "qdeq_guess_stats",
+    "qdeq_guess_inject", "qdeq_guess_infer",
 )
 TIME_OFF, TIME_UP0, TIME_TAIL, TIME_DOWN1 = 0, 1, 2, 3
+GUESS_OFF, GUESS_HARVEST, GUESS_PREDICT = 0, 1, 2
 
 
 class QdeqError(RuntimeError):
@@ -78,6 +87,23 @@ class StepInfo(ctypes.Structure):
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class GuessConfig(ctypes.Structure):
+    _fields_ = [
+        ("nk", ctypes.c_int), ("factor", ctypes.c_int), ("channels", ctypes.c_int),
+        ("width", ctypes.c_int), ("hidden", ctypes.c_int), ("capacity", ctypes.c_int),
+        ("stride", ctypes.c_int), ("seed", ctypes.c_ulonglong), ("device", ctypes.c_int),
+    ]
+
+
+class TrainConfig(ctypes.Structure):
+    _fields_ = [
+        ("epochs", ctypes.c_int), ("batch", ctypes.c_int), ("lr", ctypes.c_double),
+        ("lr_final", ctypes.c_double), ("beta1", ctypes.c_double),
+        ("beta2", ctypes.c_double), ("eps", ctypes.c_double),
+        ("seed", ctypes.c_ulonglong),
+    ]
 
 
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -129,6 +155,31 @@ def load(path: str = LIB_PATH):
     L.qdeq_rk43_step.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(StepInfo)]
     L.qdeq_rk43_get_state.argtypes = [_vp, ctypes.POINTER(StepInfo), _dp, _dp,
                                       _dp, _dp, _dp, _dp]
+    _fp = ctypes.POINTER(ctypes.c_float)
+    _llp = ctypes.POINTER(ctypes.c_longlong)
+    L.qdeq_guess_default_config.argtypes = [ctypes.POINTER(GuessConfig), ctypes.c_int]
+    L.qdeq_guess_default_config.restype = None
+    L.qdeq_train_default_config.argtypes = [ctypes.POINTER(TrainConfig)]
+    L.qdeq_train_default_config.restype = None
+    L.qdeq_guess_create.argtypes = [ctypes.POINTER(GuessConfig), ctypes.POINTER(_vp)]
+    L.qdeq_guess_destroy.argtypes = [_vp]
+    L.qdeq_guess_last_error.argtypes = [_vp]
+    L.qdeq_guess_last_error.restype = ctypes.c_char_p
+    L.qdeq_guess_num_params.argtypes = [_vp]
+    L.qdeq_guess_num_params.restype = ctypes.c_longlong
+    L.qdeq_guess_dims.argtypes = [_vp, _ip, _ip]
+    L.qdeq_guess_get_params.argtypes = [_vp, _fp]
+    L.qdeq_guess_set_params.argtypes = [_vp, _fp]
+    L.qdeq_guess_pairs.argtypes = [_vp, _ip]
+    L.qdeq_guess_clear_pairs.argtypes = [_vp]
+    L.qdeq_guess_read_pairs.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _fp, _fp, _dp]
+    L.qdeq_guess_train.argtypes = [_vp, ctypes.POINTER(TrainConfig), _dp]
+    L.qdeq_guess_forward_backward.argtypes = [_vp, ctypes.c_int, _fp, _fp, _dp, _fp]
+    L.qdeq_guess_forward.argtypes = [_vp, ctypes.c_int, _fp, _fp]
+    L.qdeq_attach_guess.argtypes = [_vp, _vp, ctypes.c_int]
+    L.qdeq_guess_stats.argtypes = [_vp, _llp, _llp, _llp]
+    L.qdeq_guess_inject.argtypes = [_vp, ctypes.c_double]
+    L.qdeq_guess_infer.argtypes = [_vp, _fp, _fp]
     for name in EXPORTS:
         getattr(L, name)  # every declared symbol must resolve
     _lib = L
@@ -137,6 +188,15 @@ def load(path: str = LIB_PATH):
 
 def _c(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _pf(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
 
 def _p(a: np.ndarray):
@@ -343,3 +403,125 @@ class Engine:
         out["U"] = U
         out["info"] = info.as_dict()
         return out
+
+
+class Guess:
+    """Learned initial guess network (include/qdeq_guess.h): owns the
+    weights, the harvested training pairs and the training loop on the GPU."""
+
+    def __init__(self, nk: int, *, device: int = 0, **overrides):
+        L = load()
+        c = GuessConfig()
+        L.qdeq_guess_default_config(ctypes.byref(c), nk)
+        c.device = device
+        for k, v in overrides.items():
+            setattr(c, k, v)
+        self.cfg = c
+        h = _vp()
+        st = L.qdeq_guess_create(ctypes.byref(c), ctypes.byref(h))
+        if st != QDEQ_OK:
+            raise QdeqError(st, L.qdeq_global_error().decode())
+        self.h = h
+        nout, f0 = ctypes.c_int(0), ctypes.c_int(0)
+        L.qdeq_guess_dims(h, ctypes.byref(nout), ctypes.byref(f0))
+        self.nk, self.nout, self.F0 = nk, nout.value, f0.value
+        self.nc = (nk - 1) // c.factor + 1
+
+    def _check(self, st: int):
+        if st != QDEQ_OK:
+            raise QdeqError(st, load().qdeq_guess_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().qdeq_guess_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def num_params(self) -> int:
+        return int(load().qdeq_guess_num_params(self.h))
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.num_params, dtype=np.float32)
+        self._check(load().qdeq_guess_get_params(self.h, _pf(out)))
+        return out
+
+    def set_params(self, flat):
+        flat = _f32(flat)
+        assert flat.size == self.num_params
+        self._check(load().qdeq_guess_set_params(self.h, _pf(flat)))
+
+    @property
+    def pairs(self) -> int:
+        n = ctypes.c_int(0)
+        self._check(load().qdeq_guess_pairs(self.h, ctypes.byref(n)))
+        return n.value
+
+    def clear_pairs(self):
+        self._check(load().qdeq_guess_clear_pairs(self.h))
+
+    def read_pairs(self, first: int, n: int):
+        x = np.zeros((n, self.nk), dtype=np.float32)
+        t = np.zeros((n, self.nout), dtype=np.float32)
+        s = np.zeros(n)
+        self._check(load().qdeq_guess_read_pairs(self.h, first, n, _pf(x), _pf(t), _p(s)))
+        return x, t, s
+
+    def train(self, **overrides) -> float:
+        tc = TrainConfig()
+        load().qdeq_train_default_config(ctypes.byref(tc))
+        for k, v in overrides.items():
+            setattr(tc, k, v)
+        loss = ctypes.c_double(0)
+        self._check(load().qdeq_guess_train(self.h, ctypes.byref(tc), ctypes.byref(loss)))
+        return loss.value
+
+    def forward_backward(self, x, t):
+        x, t = _f32(x), _f32(t)
+        B = x.shape[0]
+        loss = ctypes.c_double(0)
+        g = np.zeros(self.num_params, dtype=np.float32)
+        self._check(load().qdeq_guess_forward_backward(
+            self.h, B, _pf(x), _pf(t), ctypes.byref(loss), _pf(g)))
+        return loss.value, g
+
+    def forward(self, x) -> np.ndarray:
+        x = _f32(x)
+        y = np.zeros((x.shape[0], self.nout), dtype=np.float32)
+        self._check(load().qdeq_guess_forward(self.h, x.shape[0], _pf(x), _pf(y)))
+        return y
+
+
+def _engine_attach_guess(self, guess: Optional[Guess], mode: int):
+    """Attaches a Guess for the RK43 driver's solves (GUESS_* mode)."""
+    self._guess = guess
+    self._check(load().qdeq_attach_guess(self.h, guess.h if guess else None, mode))
+
+
+def _engine_guess_stats(self) -> dict:
+    p, a, h = ctypes.c_longlong(0), ctypes.c_longlong(0), ctypes.c_longlong(0)
+    self._check(load().qdeq_guess_stats(self.h, ctypes.byref(p), ctypes.byref(a),
+                                        ctypes.byref(h)))
+    return dict(predicted=p.value, accepted=a.value, harvested=h.value)
+
+
+def _engine_guess_inject(self, factor: float):
+    self._check(load().qdeq_guess_inject(self.h, factor))
+
+
+def _engine_guess_infer(self, x) -> np.ndarray:
+    x = _f32(x)
+    y = np.zeros(self._guess.nout, dtype=np.float32)
+    self._check(load().qdeq_guess_infer(self.h, _pf(x), _pf(y)))
+    return y
+
+
+Engine.attach_guess = _engine_attach_guess
+Engine.guess_stats = _engine_guess_stats
+Engine.guess_inject = _engine_guess_inject
+Engine.guess_infer = _engine_guess_infer

@@ -11,10 +11,16 @@ from qdeqcycle_fdm_amd import lib
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
+HEADERS = ("qdeq.h", "qdeq_guess.h")
+
+
 def _declared():
-    text = open(os.path.join(ROOT, "include", "qdeq.h")).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(qdeq_[a-z0-9_]+)\s*\(", text)))
+    names = set()
+    for h in HEADERS:
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(qdeq_[a-z0-9_]+)\s*\(", text))
+    return sorted(names)
 
 
 def test_header_declarations_match_binding_list():
@@ -47,8 +53,10 @@ def test_host_only_entry_points():
 
 
 def test_header_compiles_as_c():
-    """include/qdeq.h is plain C (no HIP or C++ types in the signatures)."""
-    src = '#include "qdeq.h"\nint main(void){qdeq_config c; qdeq_default_config(&c); return 0;}\n'
+    """include/*.h are plain C (no HIP or C++ types in the signatures)."""
+    src = ('#include "qdeq.h"\n#include "qdeq_guess.h"\n'
+           'int main(void){qdeq_config c; qdeq_default_config(&c);'
+           ' qdeq_guess_config g; qdeq_guess_default_config(&g, 1025); return 0;}\n')
     p = subprocess.run(["gcc", "-std=c99", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
                         "-x", "c", "-"], input=src, capture_output=True, text=True)
     assert p.returncode == 0, p.stderr
