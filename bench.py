@@ -15,6 +15,18 @@ kernels, exact coarsest solve), ir32 (FP32 correction cycles in FP64
 iterative refinement) and ref64 (the reference's own V-cycle: 32 sweeps per
 level, FP64).  One JSON line goes to stdout; progress goes to stderr.
 
+Learned vs plain: after the plain fast64 run, pairs are harvested from
+--harvest-runs runs of the same workload with seeded Dc / sigma_n
+perturbations, the network is trained on the GPU (both untimed, their times
+reported), and the fast64 run is repeated with the learned initial guess
+from the same start to the end of a common window: the first
+--window-steps timed plain steps, which end in the quiet phase after the
+first event (the 3000-step run itself ends inside the second event, whose
+onset shifts by ~1e-5 of the elapsed time with any change of the solves'
+roundoff, so its end state is not comparable).  V-cycles per solve, steps
+per second, simulated seconds per second and the slip difference at the
+window's end are reported for both; `value` is the plain run's.
+
 The CPU baseline is the project's FP64 port of the solver (oracle/) on one
 host core, restarted from the GPU's state at the end of the timed region and
 run for about --cpu-seconds; the GPU then runs the same steps and the two
@@ -56,11 +68,11 @@ def sync():
     lib.device_synchronize(0)
 
 
-def make_engine(pb, me, mode):
+def make_engine(pb, me, mode, **extra):
     from qdeqcycle_fdm_amd import lib
     kw = dict(MODES[mode])
     tiles = kw.pop("tiles")
-    eng = lib.Engine(pb, me, **kw)
+    eng = lib.Engine(pb, me, **kw, **extra)
     if tiles:
         eng.set_tiles(True, *tiles)
     return eng
@@ -76,7 +88,10 @@ def kernel_bytes(mode, n0, nu):
     return per * s * n0
 
 
-def run_mode(pb, me, mode, warmup, steps, timed_kernel=None):
+def run_mode(pb, me, mode, warmup, steps, timed_kernel=None, split_at=None):
+    """Warm-up, then `steps` timed accepted steps.  With split_at, the state
+    after split_at timed steps is copied out between two timed segments (the
+    copy is not timed) and the first segment's figures are kept apart."""
     from qdeqcycle_fdm_amd import lib
     eng = make_engine(pb, me, mode)
     eng.rk43_setup()
@@ -86,20 +101,99 @@ def run_mode(pb, me, mode, warmup, steps, timed_kernel=None):
     if timed_kernel is not None:
         eng.set_kernel_timing(timed_kernel)
     i0 = eng.state()["info"]
-    t0 = time.perf_counter()
-    info = eng.rk43_step(steps)
-    sync()
-    t1 = time.perf_counter()
-    out = dict(seconds=t1 - t0, steps=steps, steps_per_s=steps / (t1 - t0),
+    segs = [steps] if not split_at or split_at >= steps else [split_at, steps - split_at]
+    seconds, infos, split_state = 0.0, [], None
+    for n, k in enumerate(segs):
+        t0 = time.perf_counter()
+        info = eng.rk43_step(k)
+        sync()
+        seconds += time.perf_counter() - t0
+        infos.append((info, seconds))
+        if n == 0 and len(segs) > 1:
+            split_state = eng.state()
+    out = dict(seconds=seconds, steps=steps, steps_per_s=steps / seconds,
                vcycles=info["vcycles"] - i0["vcycles"],
                solves=info["solves"] - i0["solves"], t_start=i0["t"],
                t_end=info["t"], V_max_end=info["V_max"])
     out["vcycles_per_solve"] = out["vcycles"] / max(out["solves"], 1)
     out["solves_per_step"] = out["solves"] / steps
+    if split_state is not None:
+        f, fs = infos[0]
+        out["window"] = dict(steps=segs[0], seconds=fs, steps_per_s=segs[0] / fs,
+                             t_start=i0["t"], t_end=f["t"],
+                             vcycles_per_solve=(f["vcycles"] - i0["vcycles"]) /
+                             max(f["solves"] - i0["solves"], 1),
+                             solves_per_step=(f["solves"] - i0["solves"]) / segs[0],
+                             sim_s_per_s=(f["t"] - i0["t"]) / fs)
+        out["window_state"] = split_state
     if timed_kernel is not None:
         ms, calls = eng.kernel_timing()
         out["kernel_ms_total"], out["kernel_calls"] = ms, calls
         eng.set_kernel_timing(lib.TIME_OFF)
+    return eng, out
+
+
+def train_guess(me, steps, runs, seed, epochs):
+    """Learned initial guess: harvests (boundary change, solved displacement
+    change) pairs from `runs` runs of the workload whose Dc and sigma_n are
+    perturbed with a seeded generator (the timed run is not among them), one
+    pair every second solve, then trains the network on the GPU."""
+    import numpy as np
+    from qdeqcycle_fdm_amd import config, lib
+    rng = np.random.default_rng(seed)
+    g = lib.Guess(1025, capacity=20000, stride=2)
+    t0 = time.perf_counter()
+    params = []
+    for _ in range(runs):
+        Dc = 0.008 * float(np.exp(rng.uniform(-0.35, 0.35)))
+        sn = 50.0 * (1.0 + float(rng.uniform(-0.15, 0.15)))
+        pb = config.make_bp1_problem(1025, basin=True, tol_mode=config.TOL_REL,
+                                     tol=1e-10, Dc=Dc, sigma_n=sn)
+        eng = make_engine(pb, me, "fast64")
+        eng.rk43_setup()
+        eng.attach_guess(g, lib.GUESS_HARVEST)
+        eng.rk43_step(steps)
+        eng.close()
+        params.append({"Dc": round(Dc, 6), "sigma_n": round(sn, 3)})
+        log(f"harvest run Dc={Dc:.4f} sigma_n={sn:.2f}: {g.pairs} pairs")
+    sync()
+    harvest_s = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    loss = g.train(epochs=epochs, batch=256)
+    sync()
+    train_s = time.perf_counter() - t1
+    log(f"trained on {g.pairs} pairs in {train_s:.1f} s, loss {loss:.3e}")
+    return g, dict(pairs=g.pairs, runs=params, harvest_s=round(harvest_s, 2),
+                   train_s=round(train_s, 2), epochs=epochs, loss=loss, seed=seed)
+
+
+def run_learned(pb, me, g, warmup, tmax):
+    """The fast FP64 run with the learned guess from t = 0: warm-up, then
+    timed until t = tmax (the end of the plain run's comparison window; the
+    last step is shortened to end there)."""
+    from qdeqcycle_fdm_amd import lib
+    eng = make_engine(pb, me, "fast64", tmax=tmax)
+    eng.rk43_setup()
+    eng.attach_guess(g, lib.GUESS_PREDICT)
+    if warmup:
+        eng.rk43_step(warmup)
+    sync()
+    i0 = eng.state()["info"]
+    s0 = eng.guess_stats()
+    t0 = time.perf_counter()
+    info = eng.rk43_step(10 ** 7)
+    sync()
+    t1 = time.perf_counter()
+    s1 = eng.guess_stats()
+    steps = info["it"] - i0["it"]
+    out = dict(seconds=t1 - t0, steps=steps, steps_per_s=steps / (t1 - t0),
+               vcycles=info["vcycles"] - i0["vcycles"],
+               solves=info["solves"] - i0["solves"], t_start=i0["t"],
+               t_end=info["t"], sim_s_per_s=(info["t"] - i0["t"]) / (t1 - t0),
+               predicted=s1["predicted"] - s0["predicted"],
+               accepted=s1["accepted"] - s0["accepted"])
+    out["vcycles_per_solve"] = out["vcycles"] / max(out["solves"], 1)
+    out["solves_per_step"] = out["solves"] / max(steps, 1)
     return eng, out
 
 
@@ -150,6 +244,12 @@ def main():
     ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned-guess harvest, training and run")
+    ap.add_argument("--harvest-runs", type=int, default=2)
+    ap.add_argument("--window-steps", type=int, default=2780,
+                    help="timed plain steps of the learned-vs-plain window")
+    ap.add_argument("--train-epochs", type=int, default=30)
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="run the V-cycle as one captured graph without the "
                          "HIP-event bracket of the level-0 up kernel (used for "
@@ -182,10 +282,45 @@ def main():
 
     timed = (lib.TIME_UP0 if args.mode != "ref64" and not args.no_kernel_timing
              else None)
-    eng, main_run = run_mode(pb, me, args.mode, args.warmup, args.steps, timed)
+    # The learned-vs-plain window ends in the quiet phase after the first
+    # event (step 2800 of the default run), where both runs can be stopped
+    # at the same time; the plain run's state there is copied out untimed.
+    split = args.window_steps if args.mode == "fast64" and not args.no_learned else None
+    eng, main_run = run_mode(pb, me, args.mode, args.warmup, args.steps, timed,
+                             split_at=split)
     log(f"{args.mode}: {main_run['steps_per_s']:.2f} steps/s, "
         f"{main_run['vcycles_per_solve']:.2f} V-cycles/solve, "
         f"t {main_run['t_start']:.4g} -> {main_run['t_end']:.4g} s")
+
+    learned = None
+    if not args.no_learned and args.mode == "fast64" and "window" in main_run:
+        import numpy as np
+        g, tinfo = train_guess(me, args.warmup + args.steps, args.harvest_runs,
+                               2024, args.train_epochs)
+        win = main_run["window"]
+        e3, lr = run_learned(pb, me, g, args.warmup, win["t_end"])
+        st3 = e3.state()
+        e3.close()
+        g.close()
+        ps = main_run["window_state"]["slip"]
+        dslip = float(np.max(np.abs(st3["slip"] - ps)) / np.max(np.abs(ps)))
+        log(f"window t {win['t_start']:.4g} -> {win['t_end']:.4g} s: plain "
+            f"{win['steps']} steps, {win['steps_per_s']:.2f} steps/s, "
+            f"{win['vcycles_per_solve']:.3f} V-cycles/solve; learned "
+            f"{lr['steps']} steps, {lr['steps_per_s']:.2f} steps/s, "
+            f"{lr['vcycles_per_solve']:.3f} V-cycles/solve, accepted "
+            f"{lr['accepted']}/{lr['predicted']}; slip rel diff at the window "
+            f"end {dslip:.2e}")
+        learned = {k: lr[k] for k in ("steps", "steps_per_s", "vcycles_per_solve",
+                                      "solves_per_step", "t_start", "t_end",
+                                      "sim_s_per_s", "predicted", "accepted")}
+        learned["plain_same_window"] = {k: win[k] for k in (
+            "steps", "steps_per_s", "vcycles_per_solve", "solves_per_step",
+            "t_start", "t_end", "sim_s_per_s")}
+        learned["ends_at_window_end"] = lr["t_end"] == win["t_end"]
+        learned["slip_max_rel_diff_vs_plain_at_window_end"] = dslip
+        learned["training"] = tinfo
+    main_run.pop("window_state", None)
 
     n0 = pb.params.ni * pb.params.nk
     nu = MODES[args.mode]["n_pre"]
@@ -241,11 +376,12 @@ def main():
                "bitwise_equal_to_gpu": cb["bitwise_equal"]}
     eng.close()
 
+    head = main_run
     line = {
-        "metric": METRIC, "value": round(main_run["steps_per_s"], 3),
+        "metric": METRIC, "value": round(head["steps_per_s"], 3),
         "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 / main_run["steps_per_s"], 4),
+        "ms_per_step": round(1e3 / head["steps_per_s"], 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32+f64" if args.mode == "ir32" else "f64",
         "data": "synthetic",
@@ -253,15 +389,23 @@ def main():
                    "residual": "1e-10 relative", "mode": args.mode,
                    "cycle": ("reference V(32,32)" if args.mode == "ref64" else
                              "V(2,2), exact 17x17 coarsest solve"),
-                   "sim_time_s": [main_run["t_start"], main_run["t_end"]]},
-        "vcycles_per_solve": round(main_run["vcycles_per_solve"], 3),
-        "solves_per_step": round(main_run["solves_per_step"], 3),
+                   "initial_guess": "previous solution",
+                   "sim_time_s": [head["t_start"], head["t_end"]]},
+        "vcycles_per_solve": round(head["vcycles_per_solve"], 3),
+        "solves_per_step": round(head["solves_per_step"], 3),
+        "vcycles_per_solve_learned_vs_plain": (
+            None if learned is None else
+            [round(learned["vcycles_per_solve"], 3),
+             round(learned["plain_same_window"]["vcycles_per_solve"], 3)]),
+        "plain": {k: main_run[k] for k in ("steps", "steps_per_s",
+                                            "vcycles_per_solve", "solves_per_step")},
+        "learned": learned,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "modes": modes,
     }
     if "ref64" in modes:
-        line["speedup_vs_ref64"] = round(main_run["steps_per_s"] /
+        line["speedup_vs_ref64"] = round(head["steps_per_s"] /
                                          modes["ref64"]["steps_per_s"], 2)
     print(json.dumps(line), flush=True)
 

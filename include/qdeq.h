@@ -185,7 +185,8 @@ double qdeq_compute_step_size(int it, double dt, double mindt, double maxdt,
  * U0(remote), psi = f0 + b ln(V0/Vini). */
 int qdeq_rk43_setup(qdeq_ctx *ctx, const double *a, const double *b,
                     const double *eta, const double *tau_n0, const double *U0);
-/* Advances nsteps accepted steps (odeSolver.cpp:1462-1633).  If info is not
+/* Advances nsteps accepted steps (odeSolver.cpp:1462-1633), fewer if t
+ * reaches tmax (the last step is shortened to end there).  If info is not
  * NULL it receives the record of the last step. */
 int qdeq_rk43_step(qdeq_ctx *ctx, int nsteps, qdeq_step_info *info);
 /* Copies the current state to host arrays (any may be NULL): U (nx*nz, with

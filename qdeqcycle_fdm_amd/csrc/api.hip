@@ -601,7 +601,8 @@ int qdeq_rk43_step(qdeq_ctx *c, int nsteps, qdeq_step_info *info) {
     if ((st = rk43_first_stage(c))) return st;
   }
   const qdeq_config &p = c->cfg;
-  for (int n = 0; n < nsteps; n++) {
+  // while (t < tmax) of the reference's driver loop (odeSolver.cpp:1462)
+  for (int n = 0; n < nsteps && c->t < p.tmax; n++) {
     c->it = c->it + 1;
     int attemptCount = 0;
     while (attemptCount < 100) {  // odeSolver.cpp:1466-1604
