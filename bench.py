@@ -68,14 +68,28 @@ def sync():
     lib.device_synchronize(0)
 
 
+TUNED = {}  # mode -> per-level tile shapes found by the autotuner
+
+
 def make_engine(pb, me, mode, **extra):
+    """Engine of a mode; tiled modes use the autotuned per-level tile shapes
+    (tuned once per mode, on the first engine, before any step)."""
     from qdeqcycle_fdm_amd import lib
     kw = dict(MODES[mode])
     tiles = kw.pop("tiles")
     eng = lib.Engine(pb, me, **kw, **extra)
     if tiles:
         eng.set_tiles(True, *tiles)
+        if AUTOTUNE:
+            t0 = time.perf_counter()
+            shapes = eng.autotune_tiles(5)
+            if mode not in TUNED:
+                TUNED[mode] = dict(shapes=shapes,
+                                   seconds=round(time.perf_counter() - t0, 3))
     return eng
+
+
+AUTOTUNE = True
 
 
 def kernel_bytes(mode, n0, nu):
@@ -244,6 +258,8 @@ def main():
     ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="default tile shapes instead of the autotuned ones")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-guess harvest, training and run")
     ap.add_argument("--harvest-runs", type=int, default=2)
@@ -258,6 +274,8 @@ def main():
                     help="only time a plain device copy of N doubles (counter "
                          "calibration) and exit")
     args = ap.parse_args()
+    global AUTOTUNE
+    AUTOTUNE = not args.no_autotune
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -390,6 +408,7 @@ def main():
                    "cycle": ("reference V(32,32)" if args.mode == "ref64" else
                              "V(2,2), exact 17x17 coarsest solve"),
                    "initial_guess": "previous solution",
+                   "tile_shapes": TUNED.get(args.mode),
                    "sim_time_s": [head["t_start"], head["t_end"]]},
         "vcycles_per_solve": round(head["vcycles_per_solve"], 3),
         "solves_per_step": round(head["solves_per_step"], 3),

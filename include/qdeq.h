@@ -105,6 +105,17 @@ int qdeq_destroy(qdeq_ctx *ctx);
 int qdeq_set_tiles(qdeq_ctx *ctx, int tiled, int tile_x, int tile_y);
 /* 1 if solves of ctx currently run the fast cycle on the tiled kernels. */
 int qdeq_uses_tiled_cycle(const qdeq_ctx *ctx);
+/* Tile-shape autotuner (no reference counterpart): for every level run by
+ * the tile kernels, times `reps` launches of the down (smoothing +
+ * restriction) and the up (prolongation + smoothing) kernel for each
+ * candidate shape (interior tile_x x tile_y, thread block 64 x rows) that
+ * fits the LDS, and keeps the fastest per level and kernel.  Solutions are
+ * identical for every shape.  The level-0 displacement is preserved, so it
+ * may be called at any time; qdeq_set_tiles discards the tuned shapes. */
+int qdeq_autotune_tiles(qdeq_ctx *ctx, int reps);
+/* Shapes in use: *nlevels tile levels, shapes[6*l .. 6*l+5] = down tile_x,
+ * tile_y, thread rows, up tile_x, tile_y, thread rows (6*12 ints). */
+int qdeq_get_tile_shapes(const qdeq_ctx *ctx, int *nlevels, int *shapes);
 
 /* Measurement hooks (no reference counterpart).  qdeq_set_kernel_timing
  * brackets one kernel of the tiled fast cycle with HIP events on the

@@ -458,7 +458,7 @@ void orc_mg_destroy(orc_mg *mg) {
  * reference counterpart): B = DH^2 L assembled on the unknowns
  * j*(K-2) + k-1 from the same coefficients the sweeps use, inverted by
  * Gauss-Jordan elimination without pivoting, applied as e = B^{-1}(DH^2 f)
- * with each row summed in column order. */
+ * (row sums split into column chunks, see orc_coarse_direct). */
 static void orc_coarse_inverse(orc_mg *mg) {
   const int l = mg->L - 1;
   const long N = mg->N[l], K = mg->K[l], W = K - 2, n = N * W;
@@ -524,9 +524,19 @@ static void orc_coarse_direct(orc_mg *mg, double *u, const double *f,
     const long j = i / W, k = i - j * W + 1;
     g[i] = DH2 * f[j * K + k];
   }
+  /* columns in Q consecutive chunks of C (Q = min(4, 1024 / n)), each chunk
+   * summed in column order from 0, the chunk sums added in chunk order */
+  const long Q = (1024 / n) < 4 ? (1024 / n < 1 ? 1 : 1024 / n) : 4;
+  const long C = (n + Q - 1) / Q;
   for (long row = 0; row < n; row++) {
     double acc = 0.0;
-    for (long col = 0; col < n; col++) acc = acc + mg->cinv[row * n + col] * g[col];
+    for (long q = 0; q < Q; q++) {
+      double part = 0.0;
+      const long c1 = (q + 1) * C < n ? (q + 1) * C : n;
+      for (long col = q * C; col < c1; col++)
+        part = part + mg->cinv[row * n + col] * g[col];
+      acc = (q == 0) ? part : acc + part;
+    }
     const long j = row / W, k = row - j * W + 1;
     u[j * K + k] = acc;
   }

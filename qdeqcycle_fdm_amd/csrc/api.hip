@@ -243,7 +243,26 @@ int qdeq_set_tiles(qdeq_ctx *c, int tiled, int tile_x, int tile_y) {
   c->tiled = tiled != 0;
   c->tile_tx = tile_x;
   c->tile_ty = tile_y;
+  c->tuned = false;
   mg_destroy_graphs(c);
+  return QDEQ_OK;
+}
+
+int qdeq_autotune_tiles(qdeq_ctx *c, int reps) {
+  if (!c || reps < 1) return set_error(c, QDEQ_ERR_ARG, "bad argument");
+  return autotune_tiles(c, reps);
+}
+
+int qdeq_get_tile_shapes(const qdeq_ctx *c, int *nlevels, int *shapes) {
+  if (!c || !nlevels || !shapes) return QDEQ_ERR_ARG;
+  const int lt = use_tiled(c) ? tail_entry_level(c) : 0;
+  *nlevels = lt;
+  for (int l = 0; l < lt; l++) {
+    const TileShape dn = c->tuned ? c->tdown[l] : TileShape{c->tile_tx, c->tile_ty, 16};
+    const TileShape up = c->tuned ? c->tup[l] : TileShape{c->tile_tx, c->tile_ty, 16};
+    const int v[6] = {dn.tx, dn.ty, dn.tyt, up.tx, up.ty, up.tyt};
+    for (int i = 0; i < 6; i++) shapes[6 * l + i] = v[i];
+  }
   return QDEQ_OK;
 }
 

@@ -107,18 +107,14 @@ __global__ __launch_bounds__(1024) void k_coarse_direct(
     T *__restrict__ u, const T *__restrict__ f, const T *__restrict__ minvT,
     int N, int K, int P, T DH2, int n) {
   __shared__ T g[1024];
+  __shared__ T part[1024];
   const int W = K - 2;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     const int j = i / W, k = i - j * W + 1;
     g[i] = DH2 * f[(size_t)j * P + k];
   }
   __syncthreads();
-  for (int row = threadIdx.x; row < n; row += blockDim.x) {
-    T acc = T(0);
-    for (int col = 0; col < n; col++) acc = acc + minvT[(size_t)col * n + row] * g[col];
-    const int j = row / W, k = row - j * W + 1;
-    u[(size_t)j * P + k] = acc;
-  }
+  coarse_matvec<T>(minvT, g, part, n, u, W, K, (size_t)P);
 }
 
 void launch_coarse_direct_fp64(qdeq_ctx *c, double *u, const double *f) {

@@ -55,7 +55,7 @@ __device__ inline void tile_load(Tile<T> &t, const T *__restrict__ u,
                                  const T *__restrict__ cz, int N, int K, int P,
                                  bool zero_u) {
   const int tx = threadIdx.x, ty = threadIdx.y;
-  for (int lr = ty; lr <= t.R; lr += TY_THREADS) {
+  for (int lr = ty; lr <= t.R; lr += blockDim.y) {
     const int j = t.j0 + lr;
     const bool jin = (j >= 0) && (j < N) && lr < t.R;
     const bool jinz = (j >= 0) && (j <= N);
@@ -90,7 +90,7 @@ __device__ inline void tile_half_sweep(Tile<T> &t, int h, int colour, int N,
                                        int K, T DH2, T relax) {
   const int cols = t.C - 2 * h;
   const int half = (cols + 1) / 2;
-  for (int lr = h + threadIdx.y; lr < t.R - h; lr += TY_THREADS) {
+  for (int lr = h + threadIdx.y; lr < t.R - h; lr += blockDim.y) {
     const int j = t.j0 + lr;
     if (j < 0 || j >= N) continue;
     const int delta = (colour + j + t.k0 + h) & 1;
@@ -170,7 +170,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_down(
   // point only) or goes to the scratch block after scz
   T *sr = HAS_F ? t.sf : t.scz + (t.R + 1) * t.LS;
   const int h = H - 1;
-  for (int lr = h + threadIdx.y; lr < t.R - h; lr += TY_THREADS) {
+  for (int lr = h + threadIdx.y; lr < t.R - h; lr += blockDim.y) {
     const int j = t.j0 + lr;
     for (int lc = h + threadIdx.x; lc < t.C - h; lc += TX_THREADS) {
       const int k = t.k0 + lc;
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_down(
   __syncthreads();
   // restriction for the coarse points inside the tile interior
   const int jc0 = (t.j0 + H) / 2, kc0 = (t.k0 + H) / 2;
-  for (int a = threadIdx.y; a < TY / 2; a += TY_THREADS) {
+  for (int a = threadIdx.y; a < TY / 2; a += blockDim.y) {
     const int jc = jc0 + a;
     if (jc >= Nc) continue;
     const int jS = (jc == 0) ? 1 : 2 * jc - 1;
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_down(
     }
   }
   // smoothed interior
-  for (int lr = H + threadIdx.y; lr < H + TY; lr += TY_THREADS) {
+  for (int lr = H + threadIdx.y; lr < H + TY; lr += blockDim.y) {
     const int j = t.j0 + lr;
     if (j >= N) continue;
     for (int lc = H + threadIdx.x; lc < H + TX; lc += TX_THREADS) {
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_up(
   tile_load<T, HAS_F>(t, uin, f, cx, cz, N, K, P, false);
   __syncthreads();
   // u += P e (prolongate_kernel, odeSolver.cpp:1017-1043), point by point
-  for (int lr = threadIdx.y; lr < t.R; lr += TY_THREADS) {
+  for (int lr = threadIdx.y; lr < t.R; lr += blockDim.y) {
     const int j = t.j0 + lr;
     if (j < 0 || j >= N) continue;
     const int cj = j >> 1, dj = j & 1;
@@ -270,7 +270,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_up(
   __syncthreads();
   tile_sweeps<T, HAS_F, EXACT>(t, nu, N, K, DH2, relax);
   double acc = 0.0;
-  for (int lr = H + threadIdx.y; lr < H + TY; lr += TY_THREADS) {
+  for (int lr = H + threadIdx.y; lr < H + TY; lr += blockDim.y) {
     const int j = t.j0 + lr;
     if (j >= N) continue;
     for (int lc = H + threadIdx.x; lc < H + TX; lc += TX_THREADS) {
@@ -371,6 +371,8 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailArgs a) {
     off += 4 * (size_t)n + a.lv[l].K;
   }
   T *Rs = base + off;  // residual scratch (entry level size)
+  T *Ps = Rs + (a.lv[0].N * a.lv[0].K > a.cn ? a.lv[0].N * a.lv[0].K : a.cn);
+  // Ps: 1024 partial sums of the exact coarsest solve
   const int tid = threadIdx.x;
   // coefficients of every tail level, and the entry level's right-hand side
   for (int l = 0; l < a.nlev; l++) {
@@ -407,13 +409,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailArgs a) {
           Rs[i] = DH2 * B[l].f[j * L.K + k];
         }
         __syncthreads();
-        for (int row = tid; row < a.cn; row += blockDim.x) {
-          T acc = T(0);
-          for (int col = 0; col < a.cn; col++)
-            acc = acc + minv[(size_t)col * a.cn + row] * Rs[col];
-          const int j = row / W, k = row - j * W + 1;
-          B[l].u[j * L.K + k] = acc;
-        }
+        coarse_matvec<T>(minv, Rs, Ps, a.cn, B[l].u, W, L.K, (size_t)L.K);
       } else {
         const bool one_wave = L.N * L.K <= 512;
         tail_smooth<T, EXACT>(B[l], L.N, L.K, DH2, relax, a.nu_coarse, one_wave);
@@ -521,6 +517,7 @@ static size_t tail_lds_bytes(const qdeq_ctx *c, int lt) {
     n += 4 * (size_t)c->lv[l].N * c->lv[l].K + c->lv[l].K;
   const size_t nr = (size_t)c->lv[lt].N * c->lv[lt].K;
   n += nr > (size_t)c->coarse_n ? nr : (size_t)c->coarse_n;
+  n += 1024;  // partial sums of the exact coarsest solve
   return n * sizeof(T);
 }
 
@@ -531,11 +528,23 @@ bool tail_fits(const qdeq_ctx *c, bool fp64) {
   return b + 4096 <= c->lds_max;
 }
 
-template <typename T>
-static void tile_geometry(const qdeq_ctx *c, int level, int *TX, int *TY) {
-  *TX = c->tile_tx;
-  *TY = c->tile_ty;
-  (void)level;
+// Shape of the down (smoothing + restriction) or up (prolongation +
+// smoothing) kernel of a level.
+static TileShape tile_shape(const qdeq_ctx *c, int level, bool down) {
+  if (c->tuned) return down ? c->tdown[level] : c->tup[level];
+  return TileShape{c->tile_tx, c->tile_ty, TY_THREADS};
+}
+
+// Whether a shape fits the kernels: even sizes, the 5 x 64 columns of
+// tile_load, and the LDS with 8 KiB to spare.
+static bool shape_fits(const qdeq_ctx *c, const TileShape &s, bool down,
+                       bool has_f, size_t esz) {
+  const int nu = down ? c->cfg.n_pre : c->cfg.n_post;
+  const int H = down ? 2 * nu + 2 : 2 * nu + 1;
+  if (s.tx < 2 || s.ty < 2 || (s.tx & 1) || (s.ty & 1)) return false;
+  if (s.tyt < 1 || s.tyt > TY_THREADS || (s.tyt & (s.tyt - 1))) return false;
+  if (s.tx + 2 * H > 5 * TX_THREADS) return false;
+  return tile_lds_bytes(s.tx, s.ty, H, has_f, esz, down) + 8192 <= c->lds_max;
 }
 
 // Largest dynamic LDS the tile kernels of this context would request.
@@ -556,20 +565,21 @@ static void launch_down(qdeq_ctx *c, LevelBufs<T> *B, int l, bool has_f,
   if (!emit_now(c)) return;
   const LevelDims &d = c->lv[l];
   const LevelDims &dc = c->lv[l + 1];
-  int TX, TY;
-  tile_geometry<T>(c, l, &TX, &TY);
+  const TileShape sh = tile_shape(c, l, true);
+  const int TX = sh.tx, TY = sh.ty;
   const int nu = c->cfg.n_pre;
   const int H = 2 * nu + 2;
   const dim3 grd((d.K + TX - 1) / TX, (d.N + TY - 1) / TY);
+  const dim3 blk(TX_THREADS, sh.tyt);
   const size_t lds = tile_lds_bytes(TX, TY, H, has_f, sizeof(T), true);
   const T DH2 = (T)(d.DH * d.DH), relax = (T)c->cfg.relax;
   if (has_f)
-    hipLaunchKernelGGL((k_down<T, true, EXACT>), grd, dim3(TX_THREADS, TY_THREADS), lds,
+    hipLaunchKernelGGL((k_down<T, true, EXACT>), grd, blk, lds,
                        c->stream, B[l].u, B[l].r, B[l].f, B[l].cx, B[l].cz, d.N,
                        d.K, d.P, DH2, relax, nu, zero_u ? 1 : 0, B[l + 1].f,
                        dc.N, dc.K, dc.P, TX, TY);
   else
-    hipLaunchKernelGGL((k_down<T, false, EXACT>), grd, dim3(TX_THREADS, TY_THREADS), lds,
+    hipLaunchKernelGGL((k_down<T, false, EXACT>), grd, blk, lds,
                        c->stream, B[l].u, B[l].r, B[l].f, B[l].cx, B[l].cz, d.N,
                        d.K, d.P, DH2, relax, nu, zero_u ? 1 : 0, B[l + 1].f,
                        dc.N, dc.K, dc.P, TX, TY);
@@ -578,22 +588,18 @@ static void launch_down(qdeq_ctx *c, LevelBufs<T> *B, int l, bool has_f,
 template <typename T, bool EXACT, bool HAS_F, int MODE>
 static void launch_up_t(qdeq_ctx *c, LevelBufs<T> *B, int l, double *partial,
                         double *u64, int *nblocks) {
-  if (nblocks) {
-    const LevelDims &d0 = c->lv[l];
-    *nblocks = ((d0.K + c->tile_tx - 1) / c->tile_tx) * ((d0.N + c->tile_ty - 1) / c->tile_ty);
-  }
-  if (!emit_now(c)) return;
   const LevelDims &d = c->lv[l];
   const LevelDims &dc = c->lv[l + 1];
-  int TX, TY;
-  tile_geometry<T>(c, l, &TX, &TY);
-  const int nu = c->cfg.n_post;
-  const int H = 2 * nu + (MODE == UP_NORM ? 1 : 0);
+  const TileShape sh = tile_shape(c, l, false);
+  const int TX = sh.tx, TY = sh.ty;
   const dim3 grd((d.K + TX - 1) / TX, (d.N + TY - 1) / TY);
   if (nblocks) *nblocks = (int)(grd.x * grd.y);
+  if (!emit_now(c)) return;
+  const int nu = c->cfg.n_post;
+  const int H = 2 * nu + (MODE == UP_NORM ? 1 : 0);
   const size_t lds = tile_lds_bytes(TX, TY, H, HAS_F, sizeof(T), false);
   const T DH2 = (T)(d.DH * d.DH), relax = (T)c->cfg.relax;
-  hipLaunchKernelGGL((k_up<T, HAS_F, EXACT, true, MODE>), grd, dim3(TX_THREADS, TY_THREADS),
+  hipLaunchKernelGGL((k_up<T, HAS_F, EXACT, true, MODE>), grd, dim3(TX_THREADS, sh.tyt),
                      lds, c->stream, B[l].r, B[l].u, B[l].f, B[l].cx, B[l].cz,
                      d.N, d.K, d.P, DH2, relax, nu, B[l + 1].u, dc.N, dc.K, dc.P,
                      TX, TY, partial, u64);
@@ -657,6 +663,109 @@ void vcycle_tiled_fp64(qdeq_ctx *c, double *partial, int *nparts) {
 
 void vcycle_tiled_fp32_ir(qdeq_ctx *c) {
   vcycle_tiled<float, false>(c, c->s, true, nullptr, c->d[0].u, nullptr);
+}
+
+// ---- tile-shape autotuner ----
+
+// Launches one tile kernel of level l (down or up, in the solve's mode).
+template <typename T, bool EXACT>
+static void launch_one(qdeq_ctx *c, LevelBufs<T> *B, int l, bool down,
+                       bool level0_has_f, double *u64) {
+  const bool hf = l > 0 || level0_has_f;
+  if (down) {
+    launch_down<T, EXACT>(c, B, l, hf, hf);
+  } else if (l > 0) {
+    launch_up_t<T, EXACT, true, UP_STORE>(c, B, l, nullptr, nullptr, nullptr);
+  } else if (u64) {
+    launch_up_t<T, EXACT, true, UP_ADD64>(c, B, 0, nullptr, u64, nullptr);
+  } else {
+    int nb = 0;
+    launch_up_t<T, EXACT, false, UP_NORM>(c, B, 0, c->partial, nullptr, &nb);
+  }
+}
+
+static const int kTuneTX[] = {32, 64, 96, 128, 192, 256};
+static const int kTuneTY[] = {2, 4, 8, 16, 32, 64};
+static const int kTuneTYT[] = {4, 8, 16};
+constexpr int kPartialCap = 16384;  // c->partial entries (level-0 norm blocks)
+
+template <typename T, bool EXACT>
+static int autotune_t(qdeq_ctx *c, LevelBufs<T> *B, bool level0_has_f,
+                      double *u64, int reps) {
+  const int lt = tail_entry_level(c);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  QDEQ_HIP(c, hipEventCreate(&e0));
+  QDEQ_HIP(c, hipEventCreate(&e1));
+  int st = QDEQ_OK;
+  for (int l = 0; l < lt && st == QDEQ_OK; l++) {
+    for (int dir = 0; dir < 2 && st == QDEQ_OK; dir++) {
+      const bool down = dir == 0;
+      const bool hf = l > 0 || level0_has_f;
+      TileShape best = down ? c->tdown[l] : c->tup[l];
+      float best_ms = 1e30f;
+      for (int tx : kTuneTX)
+        for (int ty : kTuneTY)
+          for (int tyt : kTuneTYT) {
+            const TileShape cand{tx, ty, tyt};
+            if (!shape_fits(c, cand, down, hf, sizeof(T))) continue;
+            if (tyt > ty + 8) continue;  // more thread rows than tile rows
+            const LevelDims &d = c->lv[l];
+            const long blocks = (long)((d.K + tx - 1) / tx) * ((d.N + ty - 1) / ty);
+            if (!down && l == 0 && !u64 && blocks > kPartialCap) continue;
+            (down ? c->tdown[l] : c->tup[l]) = cand;
+            launch_one<T, EXACT>(c, B, l, down, level0_has_f, u64);  // warm-up
+            if (hipEventRecord(e0, c->stream) != hipSuccess) { st = QDEQ_ERR_HIP; break; }
+            for (int r = 0; r < reps; r++)
+              launch_one<T, EXACT>(c, B, l, down, level0_has_f, u64);
+            if (hipEventRecord(e1, c->stream) != hipSuccess ||
+                hipEventSynchronize(e1) != hipSuccess) { st = QDEQ_ERR_HIP; break; }
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { st = QDEQ_ERR_HIP; break; }
+            if (ms < best_ms) {
+              best_ms = ms;
+              best = cand;
+            }
+          }
+      (down ? c->tdown[l] : c->tup[l]) = best;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st != QDEQ_OK)
+    return set_error(c, st, std::string("autotune: ") + hipGetErrorString(hipGetLastError()));
+  QDEQ_HIP(c, hipGetLastError());
+  return QDEQ_OK;
+}
+
+// Times every candidate shape of each tile level's down and up kernel on the
+// context's own buffers and keeps the fastest.  Any shape gives the same
+// results; the level-0 displacement is saved and restored around the runs.
+int autotune_tiles(qdeq_ctx *c, int reps) {
+  if (!use_tiled(c))
+    return set_error(c, QDEQ_ERR_STATE, "autotune: the tiled fast cycle is not in use");
+  const LevelDims &d0 = c->lv[0];
+  const size_t n0 = (size_t)d0.N * d0.P;
+  if ((size_t)2 * c->nx * c->nz < n0)
+    return set_error(c, QDEQ_ERR_STATE, "autotune: staging buffer too small");
+  QDEQ_HIP(c, hipMemcpyAsync(c->stage_U, c->d[0].u, n0 * sizeof(double),
+                             hipMemcpyDeviceToDevice, c->stream));
+  // start from the current default shapes
+  for (int l = 0; l < QDEQ_MAX_LEVELS; l++) {
+    c->tdown[l] = c->tup[l] = TileShape{c->tile_tx, c->tile_ty, TY_THREADS};
+  }
+  c->tuned = true;
+  c->emit_phase = PH_ALL;
+  int st;
+  if (c->cfg.precision == QDEQ_PREC_FP32_IR)
+    st = autotune_t<float, false>(c, c->s, true, c->d[0].u, reps);
+  else
+    st = autotune_t<double, true>(c, c->d, false, nullptr, reps);
+  QDEQ_HIP(c, hipMemcpyAsync(c->d[0].u, c->stage_U, n0 * sizeof(double),
+                             hipMemcpyDeviceToDevice, c->stream));
+  QDEQ_HIP(c, hipStreamSynchronize(c->stream));
+  mg_destroy_graphs(c);
+  if (st != QDEQ_OK) c->tuned = false;
+  return st;
 }
 
 int tiled_setup(qdeq_ctx *c) {

@@ -24,6 +24,12 @@ struct LevelDims {
   double DH;     // grid spacing of the level
 };
 
+// Execution shape of one tile kernel launch: interior tile tx x ty points,
+// thread block 64 x tyt.
+struct TileShape {
+  int tx, ty, tyt;
+};
+
 // Device buffers of one level.  u: level 0 = displacement (FP64 path) or the
 // IR correction (FP32 path); levels >= 1 = coarse correction.  f: right-hand
 // side (level 0 FP64 = zero, never read).  r: residual scratch.  cx: west-face
@@ -86,6 +92,10 @@ struct qdeq_ctx {
   // fast cycle: tiled kernels and their interior tile shape (even sizes)
   bool tiled = true;
   int tile_tx = 64, tile_ty = 32;
+  // per-level shapes chosen by qdeq_autotune_tiles (tuned), else the above
+  // with 16 thread rows
+  bool tuned = false;
+  qdeq::TileShape tdown[QDEQ_MAX_LEVELS] = {}, tup[QDEQ_MAX_LEVELS] = {};
   size_t lds_max = 65536;  // dynamic LDS per workgroup (queried at creation)
   // exact coarsest-level solve of the fast cycle (n_coarse == 0): inverse of
   // the coarsest operator, column-major, FP64 and FP32 copies
@@ -142,6 +152,7 @@ void vcycle_tiled_fp64(qdeq_ctx *c, double *partial, int *nparts);
 void vcycle_tiled_fp32_ir(qdeq_ctx *c);
 int tiled_setup(qdeq_ctx *c);
 bool tiles_fit(const qdeq_ctx *c, int tx, int ty);
+int autotune_tiles(qdeq_ctx *c, int reps);
 
 // Kernel timing splits the captured iteration into three graphs around the
 // timed kernel; launches are emitted only in their own phase when a part is
@@ -155,7 +166,44 @@ inline void timing_mark(qdeq_ctx *c, int which, int pos) {
   if (c->timing == which) c->cur_phase = pos ? PH_POST : PH_TIMED;
 }
 
-// exact coarse solve (coarse.hip)
+// exact coarse solve (coarse.hip).  The matrix-vector product splits each
+// row's columns into coarse_chunks(n) consecutive chunks of
+// coarse_chunk_len(n) columns, summed in column order from zero, and adds
+// the chunk sums in chunk order (one thread per row and chunk, <= 1024).
+__host__ __device__ inline int coarse_chunks(int n) {
+  const int q = 1024 / n;
+  return q < 1 ? 1 : (q > 4 ? 4 : q);
+}
+__host__ __device__ inline int coarse_chunk_len(int n) {
+  const int q = coarse_chunks(n);
+  return (n + q - 1) / q;
+}
+// e[row] (rows < n, all threads of the block) = sum_col minvT[col*n+row] g[col]
+// in the chunked order; part: >= 1024 LDS elements; ends with a barrier.
+template <typename T>
+__device__ inline void coarse_matvec(const T *__restrict__ minvT, const T *g,
+                                     T *part, int n, T *out, int W, int K,
+                                     size_t pitch) {
+  const int Q = coarse_chunks(n), C = coarse_chunk_len(n);
+  for (int t = threadIdx.x; t < Q * n; t += blockDim.x) {
+    const int q = t / n, row = t - q * n;
+    const int c0 = q * C, c1 = (q + 1) * C < n ? (q + 1) * C : n;
+    T acc = T(0);
+#pragma unroll 8
+    for (int col = c0; col < c1; col++)
+      acc = acc + minvT[(size_t)col * n + row] * g[col];
+    part[t] = acc;
+  }
+  __syncthreads();
+  for (int row = threadIdx.x; row < n; row += blockDim.x) {
+    T acc = part[row];
+    for (int q = 1; q < Q; q++) acc = acc + part[q * n + row];
+    const int j = row / W, k = row - j * W + 1;
+    out[(size_t)j * pitch + k] = acc;
+  }
+  __syncthreads();
+}
+
 int build_coarse_inverse(qdeq_ctx *c);
 void launch_coarse_direct_fp64(qdeq_ctx *c, double *u, const double *f);
 void launch_coarse_direct_fp32(qdeq_ctx *c, float *u, const float *f);

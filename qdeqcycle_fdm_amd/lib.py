@@ -39,7 +39,7 @@ EXPORTS = (
     "qdeq_compute_step_size", "qdeq_rk43_setup", "qdeq_rk43_step",
     "qdeq_rk43_get_state", "qdeq_set_tiles", "qdeq_uses_tiled_cycle",
     "qdeq_set_kernel_timing", "qdeq_get_kernel_timing", "qdeq_copy_probe",
-    "qdeq_device_synchronize",
+    "qdeq_device_synchronize", "qdeq_autotune_tiles", "qdeq_get_tile_shapes",
     # include/qdeq_guess.h
     "qdeq_guess_default_config", "qdeq_train_default_config",
     "qdeq_guess_create", "qdeq_guess_destroy", "qdeq_guess_last_error",
@@ -147,6 +147,8 @@ def load(path: str = LIB_PATH):
     L.qdeq_compute_step_size.restype = ctypes.c_double
     L.qdeq_set_tiles.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.qdeq_uses_tiled_cycle.argtypes = [_vp]
+    L.qdeq_autotune_tiles.argtypes = [_vp, ctypes.c_int]
+    L.qdeq_get_tile_shapes.argtypes = [_vp, _ip, _ip]
     L.qdeq_set_kernel_timing.argtypes = [_vp, ctypes.c_int]
     L.qdeq_get_kernel_timing.argtypes = [_vp, _dp, ctypes.POINTER(ctypes.c_longlong)]
     L.qdeq_copy_probe.argtypes = [_vp, ctypes.c_longlong, ctypes.c_int, _dp]
@@ -302,6 +304,21 @@ class Engine:
     def set_tiles(self, tiled: bool = True, tile_x: int = 64, tile_y: int = 32):
         """Fast-cycle execution shape (identical results either way)."""
         self._check(load().qdeq_set_tiles(self.h, 1 if tiled else 0, tile_x, tile_y))
+
+    def autotune_tiles(self, reps: int = 5):
+        """Per-level tile shapes of the fast cycle, fastest of the candidates
+        (identical results for every shape)."""
+        self._check(load().qdeq_autotune_tiles(self.h, reps))
+        return self.tile_shapes()
+
+    def tile_shapes(self):
+        """[(down (tx, ty, rows), up (tx, ty, rows)) per tile level]."""
+        n = ctypes.c_int(0)
+        buf = (ctypes.c_int * 72)()
+        self._check(load().qdeq_get_tile_shapes(self.h, ctypes.byref(n), buf))
+        return [((buf[6 * l], buf[6 * l + 1], buf[6 * l + 2]),
+                 (buf[6 * l + 3], buf[6 * l + 4], buf[6 * l + 5]))
+                for l in range(n.value)]
 
     @property
     def uses_tiled_cycle(self) -> bool:

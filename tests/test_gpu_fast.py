@@ -137,3 +137,35 @@ def test_rk43_fast_fp64_matches_oracle(n, basin, levels, steps):
     ar = sim.arrays()
     for key in ("slip", "psi", "V", "U"):
         np.testing.assert_array_equal(st[key], ar[key], err_msg=key)
+
+
+@pytest.mark.parametrize("precision", [config.PREC_FP64, config.PREC_FP32_IR])
+def test_autotuned_tiles_give_identical_solutions(precision):
+    """The tile-shape autotuner only changes the launch shapes: the solve and
+    the RK43 steps after tuning are bit-identical to the default shapes'."""
+    pb = config.make_bp1_problem(257, basin=True, levels=5,
+                                 tol_mode=config.TOL_REL, tol=1e-10)
+    me = oracle.mu_equi(pb.mu)
+    U = _perturbed(pb, 9)
+    out = []
+    for tune in (False, True):
+        with lib.Engine(pb, me, **{**FAST, "precision": precision}) as eng:
+            eng.set_tiles(True, 64, 16)
+            if tune:
+                shapes = eng.autotune_tiles(3)
+                assert len(shapes) == 3          # 257, 129, 65 run as tiles
+                for down, up in shapes:
+                    for tx, ty, rows in (down, up):
+                        assert tx % 2 == 0 and ty % 2 == 0 and rows in (4, 8, 16)
+            V = U.copy()
+            vc = eng.ode_solve(V)[0]
+            eng.rk43_setup()
+            info = eng.rk43_step(25)
+            st = eng.state()
+            out.append((V, vc, info, st))
+    (V0, vc0, i0, s0), (V1, vc1, i1, s1) = out
+    np.testing.assert_array_equal(V0, V1)
+    assert vc0 == vc1
+    assert i0["t"] == i1["t"] and i0["vcycles"] == i1["vcycles"]
+    for k in ("slip", "psi", "U"):
+        np.testing.assert_array_equal(s0[k], s1[k])
