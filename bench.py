@@ -77,7 +77,8 @@ def make_engine(pb, me, mode, **extra):
     from qdeqcycle_fdm_amd import lib
     kw = dict(MODES[mode])
     tiles = kw.pop("tiles")
-    eng = lib.Engine(pb, me, **kw, **extra)
+    kw.update(extra)
+    eng = lib.Engine(pb, me, **kw)
     if tiles:
         eng.set_tiles(True, *tiles)
         if AUTOTUNE:

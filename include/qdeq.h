@@ -116,6 +116,12 @@ int qdeq_autotune_tiles(qdeq_ctx *ctx, int reps);
 /* Shapes in use: *nlevels tile levels, shapes[6*l .. 6*l+5] = down tile_x,
  * tile_y, thread rows, up tile_x, tile_y, thread rows (6*12 ints). */
 int qdeq_get_tile_shapes(const qdeq_ctx *ctx, int *nlevels, int *shapes);
+/* Average duration (microseconds) of each kernel of the tiled cycle, each
+ * launched `reps` times back to back on the context's buffers (the level-0
+ * displacement is preserved): *n = 2*levels+1 entries, the down kernels of
+ * the tile levels from the finest, the tail, the up kernels back to level 0
+ * (us: up to 2*12+1 doubles). */
+int qdeq_kernel_breakdown(qdeq_ctx *ctx, int reps, int *n, double *us);
 
 /* Measurement hooks (no reference counterpart).  qdeq_set_kernel_timing
  * brackets one kernel of the tiled fast cycle with HIP events on the

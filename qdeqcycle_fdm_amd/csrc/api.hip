@@ -88,6 +88,7 @@ static void free_ctx(qdeq_ctx *c) {
   for (auto &p : c->stage_v) (void)hipFree(p);
   (void)hipFree(c->fb.base);
   if (c->scal_host) (void)hipHostFree(c->scal_host);
+  if (c->mbox_host) (void)hipHostFree(c->mbox_host);
   if (c->flags_host) (void)hipHostFree(c->flags_host);
   if (c->ev_t0) (void)hipEventDestroy(c->ev_t0);
   if (c->ev_t1) (void)hipEventDestroy(c->ev_t1);
@@ -181,6 +182,9 @@ int qdeq_create(const qdeq_config *cfg, const double *mu,
   CREATE_HIP(alloc_zero(&c->scal, SC_COUNT, c->stream));
   CREATE_HIP(alloc_zero(&c->flags, 4, c->stream));
   CREATE_HIP(hipHostMalloc(&c->scal_host, SC_COUNT * sizeof(double)));
+  CREATE_HIP(hipHostMalloc(&c->mbox_host, SC_COUNT * sizeof(double),
+                           hipHostMallocMapped | hipHostMallocCoherent));
+  CREATE_HIP(hipHostGetDevicePointer((void **)&c->mbox_dev, c->mbox_host, 0));
   CREATE_HIP(hipHostMalloc(&c->flags_host, 4 * sizeof(int)));
   CREATE_HIP(alloc_zero(&c->stage_U, (size_t)2 * c->nx * c->nz, c->stream));
   for (auto &p : c->stage_v) CREATE_HIP(alloc_zero(&p, (size_t)c->nz, c->stream));
@@ -251,6 +255,11 @@ int qdeq_set_tiles(qdeq_ctx *c, int tiled, int tile_x, int tile_y) {
 int qdeq_autotune_tiles(qdeq_ctx *c, int reps) {
   if (!c || reps < 1) return set_error(c, QDEQ_ERR_ARG, "bad argument");
   return autotune_tiles(c, reps);
+}
+
+int qdeq_kernel_breakdown(qdeq_ctx *c, int reps, int *n, double *us) {
+  if (!c || reps < 1 || !n || !us) return set_error(c, QDEQ_ERR_ARG, "bad argument");
+  return kernel_breakdown(c, reps, n, us);
 }
 
 int qdeq_get_tile_shapes(const qdeq_ctx *c, int *nlevels, int *shapes) {

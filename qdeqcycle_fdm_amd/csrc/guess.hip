@@ -373,6 +373,7 @@ int guess_before_solve(qdeq_ctx *c) {
       return set_error(c, QDEQ_ERR_HIP,
                        std::string("guess graph capture: ") + hipGetErrorString(e));
   }
+  mbox_arm(c, SC_NORM_PRED);
   QDEQ_HIP(c, hipGraphLaunch(c->g_pred, c->stream));
   c->cand_pending = true;
   c->g_predicted++;

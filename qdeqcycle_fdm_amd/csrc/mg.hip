@@ -9,6 +9,8 @@
 // Sums are formed in the reference's order and the build uses
 // -ffp-contract=off, so the FP64 reference cycle reproduces the reference's
 // iterates bit for bit.
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <vector>
 
@@ -194,13 +196,20 @@ __global__ __launch_bounds__(NORM_THREADS) void k_bdres_partial(
 }
 
 // Pass 2: one block sums the partials in a fixed order, writes sqrt.
+// sqrt of the sum of n partials into out[0] and, when given, into the
+// host-mapped mailbox slot mb[0] (read by the host without a stream sync).
 __global__ __launch_bounds__(NORM_THREADS) void k_norm_final(
-    const double *__restrict__ partial, int n, double *__restrict__ out) {
+    const double *__restrict__ partial, int n, double *__restrict__ out,
+    double *__restrict__ mb) {
   __shared__ double sh[NORM_THREADS];
   double acc = 0.0;
   for (int i = threadIdx.x; i < n; i += NORM_THREADS) acc += partial[i];
   const double s = block_sum_256(acc, sh);
-  if (threadIdx.x == 0) out[0] = sqrt(s);
+  if (threadIdx.x == 0) {
+    const double v = sqrt(s);
+    out[0] = v;
+    if (mb) mb[0] = v;
+  }
 }
 
 // IR update u += e (interior columns), FP64 += FP32.
@@ -342,7 +351,7 @@ static void launch_resnorm(qdeq_ctx *c, bool store32) {
                        c->d[0].cz, d.N, d.K, d.P, DH2, c->partial,
                        (float *)nullptr, (float *)nullptr);
   hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(NORM_THREADS), 0, c->stream,
-                     c->partial, NORM_BLOCKS, c->scal + SC_NORM);
+                     c->partial, NORM_BLOCKS, c->scal + SC_NORM, c->mbox_dev + SC_NORM);
 }
 
 // Residual norm of an arbitrary level-0 displacement u (same operator and
@@ -354,7 +363,7 @@ void launch_resnorm_of(qdeq_ctx *c, const double *u, int off, int slot) {
                      d.N, d.K, d.P, d.DH * d.DH, c->partial + off,
                      (float *)nullptr, (float *)nullptr);
   hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(NORM_THREADS), 0, c->stream,
-                     c->partial + off, NORM_BLOCKS, c->scal + slot);
+                     c->partial + off, NORM_BLOCKS, c->scal + slot, c->mbox_dev + slot);
 }
 
 static void launch_bdres(qdeq_ctx *c) {
@@ -364,7 +373,8 @@ static void launch_bdres(qdeq_ctx *c) {
                      c->stream, c->d[0].u, c->d[0].cx, d.N, d.K, d.P,
                      d.DH * d.DH, c->partial + NORM_BLOCKS);
   hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(NORM_THREADS), 0, c->stream,
-                     c->partial + NORM_BLOCKS, nb, c->scal + SC_NORM0);
+                     c->partial + NORM_BLOCKS, nb, c->scal + SC_NORM0,
+                     c->mbox_dev + SC_NORM0);
 }
 
 // Whether this context runs the fast cycle with the tiled kernels.
@@ -397,7 +407,7 @@ static void launch_iteration(qdeq_ctx *c) {
     c->cur_phase = PH_POST;
     if (!emit_now(c)) return;
     hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(NORM_THREADS), 0, c->stream,
-                       c->partial, nparts, c->scal + SC_NORM);
+                       c->partial, nparts, c->scal + SC_NORM, c->mbox_dev + SC_NORM);
   } else {
     vcycle<double, true>(c, c->d, 0, false);
     launch_resnorm(c, false);
@@ -443,24 +453,63 @@ static int ensure_graph(qdeq_ctx *c) {
   return QDEQ_OK;
 }
 
+// Mailbox: k_norm_final writes each norm also into host-mapped memory, so
+// the host reads it by polling instead of a copy and a stream sync.  A slot
+// is armed with a value no norm takes (norms are >= 0 or NaN) before the
+// launches that write it.
+constexpr double kMboxEmpty = -1.0;
+
+void mbox_arm(qdeq_ctx *c, int slot) {
+  reinterpret_cast<volatile double *>(c->mbox_host)[slot] = kMboxEmpty;
+}
+
+int mbox_wait(qdeq_ctx *c, int slot) {
+  volatile double *p = reinterpret_cast<volatile double *>(c->mbox_host) + slot;
+  long spins = 0;
+  std::chrono::steady_clock::time_point idle_since{};
+  bool idle = false;
+  while (*p == kMboxEmpty) {
+    if ((++spins & 1023) != 0) continue;
+    // surface launch failures instead of spinning forever
+    const hipError_t e = hipStreamQuery(c->stream);
+    if (e != hipSuccess && e != hipErrorNotReady)
+      return set_error(c, QDEQ_ERR_HIP,
+                       std::string("waiting for a norm: ") + hipGetErrorString(e));
+    if (e == hipSuccess) {
+      // the stream is idle: the value lands shortly or was never written
+      const auto now = std::chrono::steady_clock::now();
+      if (!idle) {
+        idle = true;
+        idle_since = now;
+      } else if (now - idle_since > std::chrono::milliseconds(200)) {
+        return set_error(c, QDEQ_ERR_STATE, "norm mailbox was not written");
+      }
+    }
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
+  return QDEQ_OK;
+}
+
 // multigrid (odeSolver.cpp:652-720) on the device-resident level-0 u whose
 // Dirichlet columns are already set.  Iterates while ||r||_2 > threshold.
 int mg_solve(qdeq_ctx *c, int *vcycles, double *res0, double *res) {
   const bool ir = c->cfg.precision == QDEQ_PREC_FP32_IR;
   int st = ensure_graph(c);
   if (st != QDEQ_OK) return st;
+  // a predicted start (learned guess) was enqueued before this solve: its
+  // candidate sits in the level-0 r buffer, its residual norm in
+  // scal[SC_NORM_PRED] (and the mailbox slot, armed before the prediction)
+  const bool cand = c->cand_pending;
+  c->cand_pending = false;
+  mbox_arm(c, SC_NORM);
+  mbox_arm(c, SC_NORM0);
   if (c->cfg.tol_mode == QDEQ_TOL_REL) launch_bdres(c);
   launch_resnorm(c, ir);
   QDEQ_HIP(c, hipGetLastError());
-  // a predicted start (learned guess) was enqueued before this solve: its
-  // candidate sits in the level-0 r buffer, its residual norm in
-  // scal[SC_NORM_PRED]
-  const bool cand = c->cand_pending;
-  c->cand_pending = false;
-  QDEQ_HIP(c, hipMemcpyAsync(c->scal_host, c->scal,
-                             (cand ? SC_NORM_PRED + 1 : 2) * sizeof(double),
-                             hipMemcpyDeviceToHost, c->stream));
-  QDEQ_HIP(c, hipStreamSynchronize(c->stream));
+  if ((st = mbox_wait(c, SC_NORM))) return st;
+  if (c->cfg.tol_mode == QDEQ_TOL_REL && (st = mbox_wait(c, SC_NORM0))) return st;
+  if (cand && (st = mbox_wait(c, SC_NORM_PRED))) return st;
+  for (int i : {SC_NORM, SC_NORM0, SC_NORM_PRED}) c->scal_host[i] = c->mbox_host[i];
   double delta = c->scal_host[SC_NORM];
   if (cand) {
     // used only if its residual is finite and below the previous solution's
@@ -480,7 +529,6 @@ int mg_solve(qdeq_ctx *c, int *vcycles, double *res0, double *res) {
   const double thr =
       c->cfg.tol_mode == QDEQ_TOL_REL ? c->cfg.tol * r0 : c->cfg.tol;
   int it = 0;
-  st = QDEQ_OK;
   while (delta > thr) {
     if (it >= c->cfg.max_vcycles) {
       st = set_error(c, QDEQ_ERR_MG_NOT_CONVERGED,
@@ -489,6 +537,7 @@ int mg_solve(qdeq_ctx *c, int *vcycles, double *res0, double *res) {
                          ")");
       break;
     }
+    mbox_arm(c, SC_NORM);
     if (c->g_timed) {
       QDEQ_HIP(c, hipGraphLaunch(c->g_part[PH_PRE], c->stream));
       QDEQ_HIP(c, hipEventRecord(c->ev_t0, c->stream));
@@ -498,17 +547,16 @@ int mg_solve(qdeq_ctx *c, int *vcycles, double *res0, double *res) {
     } else {
       QDEQ_HIP(c, hipGraphLaunch(c->g_cycle, c->stream));
     }
-    QDEQ_HIP(c, hipMemcpyAsync(c->scal_host, c->scal, sizeof(double),
-                               hipMemcpyDeviceToHost, c->stream));
-    QDEQ_HIP(c, hipStreamSynchronize(c->stream));
+    if ((st = mbox_wait(c, SC_NORM))) return st;
     if (c->g_timed) {
       float ms = 0.0f;
+      QDEQ_HIP(c, hipEventSynchronize(c->ev_t1));
       if (hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1) == hipSuccess) {
         c->timed_ms += ms;
         c->timed_calls += 1;
       }
     }
-    delta = c->scal_host[SC_NORM];
+    delta = c->scal_host[SC_NORM] = c->mbox_host[SC_NORM];
     it++;
     if (!std::isfinite(delta)) {
       st = set_error(c, QDEQ_ERR_MG_NOT_CONVERGED,

@@ -768,6 +768,66 @@ int autotune_tiles(qdeq_ctx *c, int reps) {
   return st;
 }
 
+// Average duration of each kernel of the tiled cycle launched `reps` times
+// back to back in its current shape: us[0 .. lt-1] = down kernels of levels
+// 0 .. lt-1, us[lt] = tail, us[lt+1 ..] = up kernels of levels lt-1 .. 0.
+template <typename T, bool EXACT>
+static int breakdown_t(qdeq_ctx *c, LevelBufs<T> *B, bool level0_has_f,
+                       double *u64, int reps, double *us) {
+  const int lt = tail_entry_level(c);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  QDEQ_HIP(c, hipEventCreate(&e0));
+  QDEQ_HIP(c, hipEventCreate(&e1));
+  int st = QDEQ_OK;
+  for (int i = 0; i < 2 * lt + 1 && st == QDEQ_OK; i++) {
+    auto launch = [&]() {
+      if (i < lt)
+        launch_one<T, EXACT>(c, B, i, true, level0_has_f, u64);
+      else if (i == lt)
+        launch_tail<T, EXACT>(c, B, lt);
+      else
+        launch_one<T, EXACT>(c, B, 2 * lt - i, false, level0_has_f, u64);
+    };
+    launch();
+    float ms = 0.f;
+    if (hipEventRecord(e0, c->stream) != hipSuccess) st = QDEQ_ERR_HIP;
+    for (int r = 0; r < reps && st == QDEQ_OK; r++) launch();
+    if (st == QDEQ_OK &&
+        (hipEventRecord(e1, c->stream) != hipSuccess ||
+         hipEventSynchronize(e1) != hipSuccess ||
+         hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+      st = QDEQ_ERR_HIP;
+    us[i] = 1e3 * (double)ms / reps;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st != QDEQ_OK) return set_error(c, st, "kernel breakdown: HIP error");
+  QDEQ_HIP(c, hipGetLastError());
+  return QDEQ_OK;
+}
+
+int kernel_breakdown(qdeq_ctx *c, int reps, int *n, double *us) {
+  if (!use_tiled(c))
+    return set_error(c, QDEQ_ERR_STATE, "breakdown: the tiled fast cycle is not in use");
+  const LevelDims &d0 = c->lv[0];
+  const size_t n0 = (size_t)d0.N * d0.P;
+  if ((size_t)2 * c->nx * c->nz < n0)
+    return set_error(c, QDEQ_ERR_STATE, "breakdown: staging buffer too small");
+  QDEQ_HIP(c, hipMemcpyAsync(c->stage_U, c->d[0].u, n0 * sizeof(double),
+                             hipMemcpyDeviceToDevice, c->stream));
+  c->emit_phase = PH_ALL;
+  *n = 2 * tail_entry_level(c) + 1;
+  int st;
+  if (c->cfg.precision == QDEQ_PREC_FP32_IR)
+    st = breakdown_t<float, false>(c, c->s, true, c->d[0].u, reps, us);
+  else
+    st = breakdown_t<double, true>(c, c->d, false, nullptr, reps, us);
+  QDEQ_HIP(c, hipMemcpyAsync(c->d[0].u, c->stage_U, n0 * sizeof(double),
+                             hipMemcpyDeviceToDevice, c->stream));
+  QDEQ_HIP(c, hipStreamSynchronize(c->stream));
+  return st;
+}
+
 int tiled_setup(qdeq_ctx *c) {
   // Per-workgroup LDS available to dynamic allocations on this device; the
   // attribute raises the default cap where the runtime needs it.

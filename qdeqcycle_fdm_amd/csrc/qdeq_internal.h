@@ -73,6 +73,8 @@ struct qdeq_ctx {
   double *partial = nullptr;   // reduction partials (2 x NORM_BLOCKS)
   double *scal = nullptr;      // qdeq::SC_* device scalars
   double *scal_host = nullptr; // pinned mirror of scal
+  double *mbox_host = nullptr; // host-mapped norm mailbox (SC_* slots)
+  double *mbox_dev = nullptr;  // its device address
   int *flags = nullptr;        // [0] NR NaN flag
   int *flags_host = nullptr;
   double *stage_U = nullptr;   // nx*nz staging for drop-in entry points
@@ -138,6 +140,8 @@ void mg_destroy_graphs(qdeq_ctx *c);
 bool use_tiled(const qdeq_ctx *c);
 
 void launch_resnorm_of(qdeq_ctx *c, const double *u, int off, int slot);
+void mbox_arm(qdeq_ctx *c, int slot);
+int mbox_wait(qdeq_ctx *c, int slot);
 
 // learned initial guess (guess.hip)
 int guess_before_solve(qdeq_ctx *c);
@@ -153,6 +157,7 @@ void vcycle_tiled_fp32_ir(qdeq_ctx *c);
 int tiled_setup(qdeq_ctx *c);
 bool tiles_fit(const qdeq_ctx *c, int tx, int ty);
 int autotune_tiles(qdeq_ctx *c, int reps);
+int kernel_breakdown(qdeq_ctx *c, int reps, int *n, double *us);
 
 // Kernel timing splits the captured iteration into three graphs around the
 // timed kernel; launches are emitted only in their own phase when a part is

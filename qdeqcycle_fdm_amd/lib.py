@@ -40,6 +40,7 @@ EXPORTS = (
     "qdeq_rk43_get_state", "qdeq_set_tiles", "qdeq_uses_tiled_cycle",
     "qdeq_set_kernel_timing", "qdeq_get_kernel_timing", "qdeq_copy_probe",
     "qdeq_device_synchronize", "qdeq_autotune_tiles", "qdeq_get_tile_shapes",
+    "qdeq_kernel_breakdown",
     # include/qdeq_guess.h
     "qdeq_guess_default_config", "qdeq_train_default_config",
     "qdeq_guess_create", "qdeq_guess_destroy", "qdeq_guess_last_error",
@@ -149,6 +150,7 @@ def load(path: str = LIB_PATH):
     L.qdeq_uses_tiled_cycle.argtypes = [_vp]
     L.qdeq_autotune_tiles.argtypes = [_vp, ctypes.c_int]
     L.qdeq_get_tile_shapes.argtypes = [_vp, _ip, _ip]
+    L.qdeq_kernel_breakdown.argtypes = [_vp, ctypes.c_int, _ip, _dp]
     L.qdeq_set_kernel_timing.argtypes = [_vp, ctypes.c_int]
     L.qdeq_get_kernel_timing.argtypes = [_vp, _dp, ctypes.POINTER(ctypes.c_longlong)]
     L.qdeq_copy_probe.argtypes = [_vp, ctypes.c_longlong, ctypes.c_int, _dp]
@@ -310,6 +312,14 @@ class Engine:
         (identical results for every shape)."""
         self._check(load().qdeq_autotune_tiles(self.h, reps))
         return self.tile_shapes()
+
+    def kernel_breakdown(self, reps: int = 20) -> list:
+        """Average microseconds of each tiled-cycle kernel launched back to
+        back: down kernels (finest first), tail, up kernels (to level 0)."""
+        n = ctypes.c_int(0)
+        buf = (ctypes.c_double * 25)()
+        self._check(load().qdeq_kernel_breakdown(self.h, reps, ctypes.byref(n), buf))
+        return [buf[i] for i in range(n.value)]
 
     def tile_shapes(self):
         """[(down (tx, ty, rows), up (tx, ty, rows)) per tile level]."""
