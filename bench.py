@@ -341,6 +341,18 @@ def main():
         learned["training"] = tinfo
     main_run.pop("window_state", None)
 
+    # per-kernel durations of the tiled cycle (each kernel launched back to
+    # back on the engine's buffers, its shape as tuned)
+    breakdown = None
+    if args.mode != "ref64":
+        us = eng.kernel_breakdown(20)
+        lt = (len(us) - 1) // 2
+        names = ([f"down{l}" for l in range(lt)] + ["tail"] +
+                 [f"up{l}" for l in range(lt - 1, -1, -1)])
+        breakdown = {n: round(v, 2) for n, v in zip(names, us)}
+        log("kernels (us, back to back): " +
+            " ".join(f"{n} {v:.1f}" for n, v in breakdown.items()))
+
     n0 = pb.params.ni * pb.params.nk
     nu = MODES[args.mode]["n_pre"]
     roofline = None
@@ -421,6 +433,7 @@ def main():
                                             "vcycles_per_solve", "solves_per_step")},
         "learned": learned,
         "roofline": roofline,
+        "kernel_breakdown_us": breakdown,
         "cpu_baseline": cpu,
         "modes": modes,
     }

@@ -126,9 +126,9 @@ int qdeq_kernel_breakdown(qdeq_ctx *ctx, int reps, int *n, double *us);
 /* Measurement hooks (no reference counterpart).  qdeq_set_kernel_timing
  * brackets one kernel of the tiled fast cycle with HIP events on the
  * context's stream (the captured V-cycle is split into three graphs around
- * it, the events recorded between their launches) and accumulates its
- * duration over every following V-cycle; qdeq_get_kernel_timing reads the
- * total and the count.  qdeq_copy_probe times `reps` launches of a plain
+ * it, the events recorded between their launches) on every 8th V-cycle
+ * and accumulates its duration; qdeq_get_kernel_timing reads the total and
+ * the count of timed launches.  qdeq_copy_probe times `reps` launches of a plain
  * device copy of n_doubles doubles (bandwidth and counter calibration). */
 enum { QDEQ_TIME_OFF = 0, QDEQ_TIME_UP0 = 1, QDEQ_TIME_TAIL = 2,
        QDEQ_TIME_DOWN1 = 3 };

@@ -294,6 +294,7 @@ int qdeq_set_kernel_timing(qdeq_ctx *c, int which) {
   c->timing = which;
   c->timed_ms = 0.0;
   c->timed_calls = 0;
+  c->timed_seq = 0;
   mg_destroy_graphs(c);
   return QDEQ_OK;
 }

@@ -430,8 +430,9 @@ static int ensure_graph(qdeq_ctx *c) {
                    8 * c->timing + 64 * (c->tile_tx * 1024 + c->tile_ty);
   if (c->g_cycle && c->g_cycle_mode == mode) return QDEQ_OK;
   mg_destroy_graphs(c);
+  // the whole iteration, and with kernel timing also its three parts
   const bool split = c->timing != 0 && use_tiled(c);
-  for (int ph = split ? PH_PRE : PH_ALL; ph <= (split ? PH_POST : PH_ALL); ph++) {
+  for (int ph = PH_ALL; ph <= (split ? PH_POST : PH_ALL); ph++) {
     hipGraph_t g = nullptr;
     c->emit_phase = ph;
     hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
@@ -441,8 +442,8 @@ static int ensure_graph(qdeq_ctx *c) {
     }
     c->emit_phase = PH_ALL;
     if (e == hipSuccess)
-      e = hipGraphInstantiate(split ? &c->g_part[ph] : &c->g_cycle, g, nullptr,
-                              nullptr, 0);
+      e = hipGraphInstantiate(ph == PH_ALL ? &c->g_cycle : &c->g_part[ph], g,
+                              nullptr, nullptr, 0);
     if (g) (void)hipGraphDestroy(g);
     if (e != hipSuccess)
       return set_error(c, QDEQ_ERR_HIP,
@@ -458,6 +459,7 @@ static int ensure_graph(qdeq_ctx *c) {
 // is armed with a value no norm takes (norms are >= 0 or NaN) before the
 // launches that write it.
 constexpr double kMboxEmpty = -1.0;
+constexpr long long kTimingEvery = 8;
 
 void mbox_arm(qdeq_ctx *c, int slot) {
   reinterpret_cast<volatile double *>(c->mbox_host)[slot] = kMboxEmpty;
@@ -538,7 +540,10 @@ int mg_solve(qdeq_ctx *c, int *vcycles, double *res0, double *res) {
       break;
     }
     mbox_arm(c, SC_NORM);
-    if (c->g_timed) {
+    // kernel timing samples every kTimingEvery-th V-cycle (the split graphs
+    // cost two extra launches)
+    const bool sample = c->g_timed && (c->timed_seq++ % kTimingEvery) == 0;
+    if (sample) {
       QDEQ_HIP(c, hipGraphLaunch(c->g_part[PH_PRE], c->stream));
       QDEQ_HIP(c, hipEventRecord(c->ev_t0, c->stream));
       QDEQ_HIP(c, hipGraphLaunch(c->g_part[PH_TIMED], c->stream));
@@ -548,7 +553,7 @@ int mg_solve(qdeq_ctx *c, int *vcycles, double *res0, double *res) {
       QDEQ_HIP(c, hipGraphLaunch(c->g_cycle, c->stream));
     }
     if ((st = mbox_wait(c, SC_NORM))) return st;
-    if (c->g_timed) {
+    if (sample) {
       float ms = 0.0f;
       QDEQ_HIP(c, hipEventSynchronize(c->ev_t1));
       if (hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1) == hipSuccess) {

@@ -112,7 +112,7 @@ struct qdeq_ctx {
   int emit_phase = 0, cur_phase = 1;
   hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;
   double timed_ms = 0.0;
-  long long timed_calls = 0;
+  long long timed_calls = 0, timed_seq = 0;
   // learned initial guess (guess.hip): attached network, mode, the response
   // W_R to unit remote data, inference scratch (16 padded rows) and the
   // boundary data of the previous solve
