@@ -360,19 +360,24 @@ def main():
         avg_s = main_run["kernel_ms_total"] / main_run["kernel_calls"] * 1e-3
         bytes_ = kernel_bytes(args.mode, n0, nu)
         achieved = bytes_ / avg_s / 1e9
+        # HBM bytes per launch from the rocprofv3 counter passes on the same
+        # kernel and tile shape (profiles/counters_basin_1025_fast64.md)
         traffic = None
         tpath = os.path.join(ROOT, "profiles", "traffic.json")
+        shape_up0 = None
+        if TUNED.get(args.mode):
+            shape_up0 = list(TUNED[args.mode]["shapes"][0][1])
         if os.path.exists(tpath):
             with open(tpath) as fh:
                 tr = json.load(fh)
-            key = f"{args.mode}:up0"
-            if key in tr:
-                traffic = tr[key]["hbm_bytes_per_launch"]
+            rec = tr.get(f"{args.mode}:up0")
+            if rec and list(rec.get("shape", [])) == shape_up0:
+                traffic = rec["hbm_bytes_per_launch"]
         roofline = {
             "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": f"k_up level 0 ({args.mode})",
+            "kernel": f"k_up level 0 ({args.mode}), tile {shape_up0}",
             "avg_launch_us": round(avg_s * 1e6, 2),
             "launches": main_run["kernel_calls"],
             "algorithmic_bytes_per_launch": bytes_,

@@ -109,13 +109,17 @@ int qdeq_uses_tiled_cycle(const qdeq_ctx *ctx);
  * the tile kernels, times `reps` launches of the down (smoothing +
  * restriction) and the up (prolongation + smoothing) kernel for each
  * candidate shape (interior tile_x x tile_y, thread block 64 x rows) that
- * fits the LDS, and keeps the fastest per level and kernel.  Solutions are
+ * fits the LDS, and keeps per level and kernel the first candidate (in a
+ * fixed order) within 3% of the fastest.  Solutions are
  * identical for every shape.  The level-0 displacement is preserved, so it
  * may be called at any time; qdeq_set_tiles discards the tuned shapes. */
 int qdeq_autotune_tiles(qdeq_ctx *ctx, int reps);
 /* Shapes in use: *nlevels tile levels, shapes[6*l .. 6*l+5] = down tile_x,
  * tile_y, thread rows, up tile_x, tile_y, thread rows (6*12 ints). */
 int qdeq_get_tile_shapes(const qdeq_ctx *ctx, int *nlevels, int *shapes);
+/* Pins the shapes (same layout; nlevels must equal the tile level count),
+ * e.g. to reuse a tuning; rejected if any does not fit the kernels. */
+int qdeq_set_tile_shapes(qdeq_ctx *ctx, int nlevels, const int *shapes);
 /* Average duration (microseconds) of each kernel of the tiled cycle, each
  * launched `reps` times back to back on the context's buffers (the level-0
  * displacement is preserved): *n = 2*levels+1 entries, the down kernels of
@@ -125,10 +129,10 @@ int qdeq_kernel_breakdown(qdeq_ctx *ctx, int reps, int *n, double *us);
 
 /* Measurement hooks (no reference counterpart).  qdeq_set_kernel_timing
  * brackets one kernel of the tiled fast cycle with HIP events on the
- * context's stream (the captured V-cycle is split into three graphs around
- * it, the events recorded between their launches) on every 8th V-cycle
- * and accumulates its duration; qdeq_get_kernel_timing reads the total and
- * the count of timed launches.  qdeq_copy_probe times `reps` launches of a plain
+ * context's stream: every 8th V-cycle is launched directly instead of as
+ * the captured graph (same kernels, same arguments) with the events recorded
+ * right before and after that kernel; qdeq_get_kernel_timing reads the
+ * accumulated duration and the count of timed launches.  qdeq_copy_probe times `reps` launches of a plain
  * device copy of n_doubles doubles (bandwidth and counter calibration). */
 enum { QDEQ_TIME_OFF = 0, QDEQ_TIME_UP0 = 1, QDEQ_TIME_TAIL = 2,
        QDEQ_TIME_DOWN1 = 3 };

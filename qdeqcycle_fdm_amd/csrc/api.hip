@@ -257,6 +257,11 @@ int qdeq_autotune_tiles(qdeq_ctx *c, int reps) {
   return autotune_tiles(c, reps);
 }
 
+int qdeq_set_tile_shapes(qdeq_ctx *c, int nlevels, const int *shapes) {
+  if (!c || !shapes) return set_error(c, QDEQ_ERR_ARG, "null argument");
+  return set_tile_shapes(c, nlevels, shapes);
+}
+
 int qdeq_kernel_breakdown(qdeq_ctx *c, int reps, int *n, double *us) {
   if (!c || reps < 1 || !n || !us) return set_error(c, QDEQ_ERR_ARG, "bad argument");
   return kernel_breakdown(c, reps, n, us);

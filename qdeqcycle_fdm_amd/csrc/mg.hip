@@ -388,12 +388,9 @@ bool use_tiled(const qdeq_ctx *c) {
 // the oracle); the FP32 correction cycle uses a plain division.
 static void launch_iteration(qdeq_ctx *c) {
   const bool tiled = use_tiled(c);
-  c->cur_phase = PH_PRE;
   if (c->cfg.precision == QDEQ_PREC_FP32_IR) {
     if (tiled) {
       vcycle_tiled_fp32_ir(c);
-      c->cur_phase = PH_POST;
-      if (!emit_now(c)) return;
     } else {
       vcycle<float, false>(c, c->s, 0, true);
       const LevelDims &d = c->lv[0];
@@ -404,8 +401,6 @@ static void launch_iteration(qdeq_ctx *c) {
   } else if (tiled) {
     int nparts = 0;
     vcycle_tiled_fp64(c, c->partial, &nparts);
-    c->cur_phase = PH_POST;
-    if (!emit_now(c)) return;
     hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(NORM_THREADS), 0, c->stream,
                        c->partial, nparts, c->scal + SC_NORM, c->mbox_dev + SC_NORM);
   } else {
@@ -417,39 +412,25 @@ static void launch_iteration(qdeq_ctx *c) {
 void mg_destroy_graphs(qdeq_ctx *c) {
   if (c->g_cycle) (void)hipGraphExecDestroy(c->g_cycle);
   c->g_cycle = nullptr;
-  for (auto &g : c->g_part) {
-    if (g) (void)hipGraphExecDestroy(g);
-    g = nullptr;
-  }
-  c->g_timed = false;
   c->g_cycle_mode = -1;
 }
 
 static int ensure_graph(qdeq_ctx *c) {
   const int mode = c->cfg.precision * 2 + c->cfg.cycle + (c->tiled ? 4 : 0) +
-                   8 * c->timing + 64 * (c->tile_tx * 1024 + c->tile_ty);
+                   64 * (c->tile_tx * 1024 + c->tile_ty);
   if (c->g_cycle && c->g_cycle_mode == mode) return QDEQ_OK;
   mg_destroy_graphs(c);
-  // the whole iteration, and with kernel timing also its three parts
-  const bool split = c->timing != 0 && use_tiled(c);
-  for (int ph = PH_ALL; ph <= (split ? PH_POST : PH_ALL); ph++) {
-    hipGraph_t g = nullptr;
-    c->emit_phase = ph;
-    hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
-    if (e == hipSuccess) {
-      launch_iteration(c);
-      e = hipStreamEndCapture(c->stream, &g);
-    }
-    c->emit_phase = PH_ALL;
-    if (e == hipSuccess)
-      e = hipGraphInstantiate(ph == PH_ALL ? &c->g_cycle : &c->g_part[ph], g,
-                              nullptr, nullptr, 0);
-    if (g) (void)hipGraphDestroy(g);
-    if (e != hipSuccess)
-      return set_error(c, QDEQ_ERR_HIP,
-                       std::string("V-cycle graph capture: ") + hipGetErrorString(e));
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    launch_iteration(c);
+    e = hipStreamEndCapture(c->stream, &g);
   }
-  c->g_timed = split;
+  if (e == hipSuccess) e = hipGraphInstantiate(&c->g_cycle, g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  if (e != hipSuccess)
+    return set_error(c, QDEQ_ERR_HIP,
+                     std::string("V-cycle graph capture: ") + hipGetErrorString(e));
   c->g_cycle_mode = mode;
   return QDEQ_OK;
 }
@@ -540,15 +521,16 @@ int mg_solve(qdeq_ctx *c, int *vcycles, double *res0, double *res) {
       break;
     }
     mbox_arm(c, SC_NORM);
-    // kernel timing samples every kTimingEvery-th V-cycle (the split graphs
-    // cost two extra launches)
-    const bool sample = c->g_timed && (c->timed_seq++ % kTimingEvery) == 0;
+    // kernel timing samples every kTimingEvery-th V-cycle: that one is
+    // launched directly (same kernels and arguments as the graph) with the
+    // events recorded around the timed kernel
+    const bool sample = c->timing != 0 && use_tiled(c) &&
+                        (c->timed_seq++ % kTimingEvery) == 0;
     if (sample) {
-      QDEQ_HIP(c, hipGraphLaunch(c->g_part[PH_PRE], c->stream));
-      QDEQ_HIP(c, hipEventRecord(c->ev_t0, c->stream));
-      QDEQ_HIP(c, hipGraphLaunch(c->g_part[PH_TIMED], c->stream));
-      QDEQ_HIP(c, hipEventRecord(c->ev_t1, c->stream));
-      QDEQ_HIP(c, hipGraphLaunch(c->g_part[PH_POST], c->stream));
+      c->emit_events = true;
+      launch_iteration(c);
+      c->emit_events = false;
+      QDEQ_HIP(c, hipGetLastError());
     } else {
       QDEQ_HIP(c, hipGraphLaunch(c->g_cycle, c->stream));
     }

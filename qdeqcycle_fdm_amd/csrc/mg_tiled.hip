@@ -18,6 +18,10 @@
 // halo).
 #include <cmath>
 
+#include <algorithm>
+#include <utility>
+#include <vector>
+
 #include "qdeq_internal.h"
 
 namespace qdeq {
@@ -562,7 +566,6 @@ bool tiles_fit(const qdeq_ctx *c, int tx, int ty) {
 template <typename T, bool EXACT>
 static void launch_down(qdeq_ctx *c, LevelBufs<T> *B, int l, bool has_f,
                         bool zero_u) {
-  if (!emit_now(c)) return;
   const LevelDims &d = c->lv[l];
   const LevelDims &dc = c->lv[l + 1];
   const TileShape sh = tile_shape(c, l, true);
@@ -594,7 +597,6 @@ static void launch_up_t(qdeq_ctx *c, LevelBufs<T> *B, int l, double *partial,
   const int TX = sh.tx, TY = sh.ty;
   const dim3 grd((d.K + TX - 1) / TX, (d.N + TY - 1) / TY);
   if (nblocks) *nblocks = (int)(grd.x * grd.y);
-  if (!emit_now(c)) return;
   const int nu = c->cfg.n_post;
   const int H = 2 * nu + (MODE == UP_NORM ? 1 : 0);
   const size_t lds = tile_lds_bytes(TX, TY, H, HAS_F, sizeof(T), false);
@@ -607,7 +609,6 @@ static void launch_up_t(qdeq_ctx *c, LevelBufs<T> *B, int l, double *partial,
 
 template <typename T, bool EXACT>
 static void launch_tail(qdeq_ctx *c, LevelBufs<T> *B, int lt) {
-  if (!emit_now(c)) return;
   TailArgs a{};
   a.nlev = c->L - lt;
   for (int i = 0; i < a.nlev; i++) {
@@ -701,8 +702,9 @@ static int autotune_t(qdeq_ctx *c, LevelBufs<T> *B, bool level0_has_f,
     for (int dir = 0; dir < 2 && st == QDEQ_OK; dir++) {
       const bool down = dir == 0;
       const bool hf = l > 0 || level0_has_f;
-      TileShape best = down ? c->tdown[l] : c->tup[l];
-      float best_ms = 1e30f;
+      // every candidate's time, in a fixed order; the first one within 3%
+      // of the fastest wins (stable against timing noise between runs)
+      std::vector<std::pair<TileShape, float>> times;
       for (int tx : kTuneTX)
         for (int ty : kTuneTY)
           for (int tyt : kTuneTYT) {
@@ -721,12 +723,16 @@ static int autotune_t(qdeq_ctx *c, LevelBufs<T> *B, bool level0_has_f,
                 hipEventSynchronize(e1) != hipSuccess) { st = QDEQ_ERR_HIP; break; }
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { st = QDEQ_ERR_HIP; break; }
-            if (ms < best_ms) {
-              best_ms = ms;
-              best = cand;
-            }
+            times.emplace_back(cand, ms);
           }
-      (down ? c->tdown[l] : c->tup[l]) = best;
+      if (times.empty()) continue;
+      float best_ms = times[0].second;
+      for (const auto &t : times) best_ms = std::min(best_ms, t.second);
+      for (const auto &t : times)
+        if (t.second <= 1.03f * best_ms) {
+          (down ? c->tdown[l] : c->tup[l]) = t.first;
+          break;
+        }
     }
   }
   (void)hipEventDestroy(e0);
@@ -734,6 +740,39 @@ static int autotune_t(qdeq_ctx *c, LevelBufs<T> *B, bool level0_has_f,
   if (st != QDEQ_OK)
     return set_error(c, st, std::string("autotune: ") + hipGetErrorString(hipGetLastError()));
   QDEQ_HIP(c, hipGetLastError());
+  return QDEQ_OK;
+}
+
+// Sets the per-level shapes explicitly (validated like tuned ones).
+int set_tile_shapes(qdeq_ctx *c, int nlevels, const int *shapes) {
+  if (!use_tiled(c))
+    return set_error(c, QDEQ_ERR_STATE, "tile shapes: the tiled fast cycle is not in use");
+  const int lt = tail_entry_level(c);
+  if (nlevels != lt)
+    return set_error(c, QDEQ_ERR_ARG, "tile shapes: need one entry per tile level (" +
+                                          std::to_string(lt) + ")");
+  const bool ir = c->cfg.precision == QDEQ_PREC_FP32_IR;
+  const size_t esz = ir ? 4 : 8;
+  TileShape dn[QDEQ_MAX_LEVELS], up[QDEQ_MAX_LEVELS];
+  for (int l = 0; l < lt; l++) {
+    dn[l] = TileShape{shapes[6 * l], shapes[6 * l + 1], shapes[6 * l + 2]};
+    up[l] = TileShape{shapes[6 * l + 3], shapes[6 * l + 4], shapes[6 * l + 5]};
+    const bool hf = l > 0 || ir;
+    if (!shape_fits(c, dn[l], true, hf, esz) || !shape_fits(c, up[l], false, hf, esz))
+      return set_error(c, QDEQ_ERR_ARG, "tile shapes: level " + std::to_string(l) +
+                                            " does not fit the kernels");
+    const LevelDims &d = c->lv[l];
+    const long blocks = (long)((d.K + up[l].tx - 1) / up[l].tx) *
+                        ((d.N + up[l].ty - 1) / up[l].ty);
+    if (l == 0 && !ir && blocks > kPartialCap)
+      return set_error(c, QDEQ_ERR_ARG, "tile shapes: too many level-0 blocks");
+  }
+  for (int l = 0; l < lt; l++) {
+    c->tdown[l] = dn[l];
+    c->tup[l] = up[l];
+  }
+  c->tuned = true;
+  mg_destroy_graphs(c);
   return QDEQ_OK;
 }
 
@@ -754,7 +793,6 @@ int autotune_tiles(qdeq_ctx *c, int reps) {
     c->tdown[l] = c->tup[l] = TileShape{c->tile_tx, c->tile_ty, TY_THREADS};
   }
   c->tuned = true;
-  c->emit_phase = PH_ALL;
   int st;
   if (c->cfg.precision == QDEQ_PREC_FP32_IR)
     st = autotune_t<float, false>(c, c->s, true, c->d[0].u, reps);
@@ -815,7 +853,6 @@ int kernel_breakdown(qdeq_ctx *c, int reps, int *n, double *us) {
     return set_error(c, QDEQ_ERR_STATE, "breakdown: staging buffer too small");
   QDEQ_HIP(c, hipMemcpyAsync(c->stage_U, c->d[0].u, n0 * sizeof(double),
                              hipMemcpyDeviceToDevice, c->stream));
-  c->emit_phase = PH_ALL;
   *n = 2 * tail_entry_level(c) + 1;
   int st;
   if (c->cfg.precision == QDEQ_PREC_FP32_IR)

@@ -105,11 +105,10 @@ struct qdeq_ctx {
   float *cinv32 = nullptr;
   int coarse_n = 0;
   // HIP-event timing of one kernel of the fast cycle (0 = off; see
-  // qdeq_set_kernel_timing): events bracket it inside the captured graph
+  // qdeq_set_kernel_timing): on sampled V-cycles the cycle is launched
+  // directly instead of as a graph (emit_events) with events around it
   int timing = 0;
-  bool g_timed = false;
-  hipGraphExec_t g_part[4] = {};   // pre / timed kernel / post graphs
-  int emit_phase = 0, cur_phase = 1;
+  bool emit_events = false;
   hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;
   double timed_ms = 0.0;
   long long timed_calls = 0, timed_seq = 0;
@@ -157,18 +156,14 @@ void vcycle_tiled_fp32_ir(qdeq_ctx *c);
 int tiled_setup(qdeq_ctx *c);
 bool tiles_fit(const qdeq_ctx *c, int tx, int ty);
 int autotune_tiles(qdeq_ctx *c, int reps);
+int set_tile_shapes(qdeq_ctx *c, int nlevels, const int *shapes);
 int kernel_breakdown(qdeq_ctx *c, int reps, int *n, double *us);
 
-// Kernel timing splits the captured iteration into three graphs around the
-// timed kernel; launches are emitted only in their own phase when a part is
-// being captured (emit_phase), and every launch when it is 0.
-enum { PH_ALL = 0, PH_PRE = 1, PH_TIMED = 2, PH_POST = 3 };
-inline bool emit_now(const qdeq_ctx *c) {
-  return c->emit_phase == PH_ALL || c->emit_phase == c->cur_phase;
-}
-// marks the start (pos 0) or end (pos 1) of kernel `which` in the sequence
+// marks the start (pos 0) or end (pos 1) of kernel `which` in the launch
+// sequence: records the timing events on a directly launched V-cycle
 inline void timing_mark(qdeq_ctx *c, int which, int pos) {
-  if (c->timing == which) c->cur_phase = pos ? PH_POST : PH_TIMED;
+  if (c->emit_events && c->timing == which)
+    (void)hipEventRecord(pos ? c->ev_t1 : c->ev_t0, c->stream);
 }
 
 // exact coarse solve (coarse.hip).  The matrix-vector product splits each

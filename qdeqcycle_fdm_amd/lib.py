@@ -40,7 +40,7 @@ EXPORTS = (
     "qdeq_rk43_get_state", "qdeq_set_tiles", "qdeq_uses_tiled_cycle",
     "qdeq_set_kernel_timing", "qdeq_get_kernel_timing", "qdeq_copy_probe",
     "qdeq_device_synchronize", "qdeq_autotune_tiles", "qdeq_get_tile_shapes",
-    "qdeq_kernel_breakdown",
+    "qdeq_kernel_breakdown", "qdeq_set_tile_shapes",
     # include/qdeq_guess.h
     "qdeq_guess_default_config", "qdeq_train_default_config",
     "qdeq_guess_create", "qdeq_guess_destroy", "qdeq_guess_last_error",
@@ -151,6 +151,7 @@ def load(path: str = LIB_PATH):
     L.qdeq_autotune_tiles.argtypes = [_vp, ctypes.c_int]
     L.qdeq_get_tile_shapes.argtypes = [_vp, _ip, _ip]
     L.qdeq_kernel_breakdown.argtypes = [_vp, ctypes.c_int, _ip, _dp]
+    L.qdeq_set_tile_shapes.argtypes = [_vp, ctypes.c_int, _ip]
     L.qdeq_set_kernel_timing.argtypes = [_vp, ctypes.c_int]
     L.qdeq_get_kernel_timing.argtypes = [_vp, _dp, ctypes.POINTER(ctypes.c_longlong)]
     L.qdeq_copy_probe.argtypes = [_vp, ctypes.c_longlong, ctypes.c_int, _dp]
@@ -320,6 +321,12 @@ class Engine:
         buf = (ctypes.c_double * 25)()
         self._check(load().qdeq_kernel_breakdown(self.h, reps, ctypes.byref(n), buf))
         return [buf[i] for i in range(n.value)]
+
+    def set_tile_shapes(self, shapes):
+        """Pins per-level shapes [((tx, ty, rows), (tx, ty, rows)), ...]."""
+        flat = [v for down, up in shapes for v in (*down, *up)]
+        buf = (ctypes.c_int * max(len(flat), 1))(*flat)
+        self._check(load().qdeq_set_tile_shapes(self.h, len(shapes), buf))
 
     def tile_shapes(self):
         """[(down (tx, ty, rows), up (tx, ty, rows)) per tile level]."""

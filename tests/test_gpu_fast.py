@@ -169,3 +169,27 @@ def test_autotuned_tiles_give_identical_solutions(precision):
     assert i0["t"] == i1["t"] and i0["vcycles"] == i1["vcycles"]
     for k in ("slip", "psi", "U"):
         np.testing.assert_array_equal(s0[k], s1[k])
+
+
+def test_pinned_tile_shapes_round_trip_and_validation():
+    pb = config.make_bp1_problem(257, basin=True, levels=5,
+                                 tol_mode=config.TOL_REL, tol=1e-10)
+    me = oracle.mu_equi(pb.mu)
+    with lib.Engine(pb, me, **FAST) as eng:
+        eng.set_tiles(True, 64, 16)
+        shapes = [((32, 8, 8), (64, 4, 4)), ((96, 16, 16), (32, 2, 8)),
+                  ((64, 8, 16), (64, 8, 16))]
+        eng.set_tile_shapes(shapes)
+        assert eng.tile_shapes() == shapes
+        with pytest.raises(lib.QdeqError):
+            eng.set_tile_shapes(shapes[:2])                        # level count
+        with pytest.raises(lib.QdeqError):
+            eng.set_tile_shapes([((33, 8, 8), (64, 4, 4))] + shapes[1:])  # odd
+        with pytest.raises(lib.QdeqError):
+            eng.set_tile_shapes([((512, 64, 16), (64, 4, 4))] + shapes[1:])  # LDS
+        assert eng.tile_shapes() == shapes
+        U = _perturbed(pb, 4)
+        V = U.copy()
+        eng.ode_solve(V)
+    V2, _ = _gpu_solve(pb, me, U, True)
+    np.testing.assert_array_equal(V, V2)
