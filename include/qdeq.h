@@ -210,6 +210,42 @@ int qdeq_rk43_setup(qdeq_ctx *ctx, const double *a, const double *b,
  * reaches tmax (the last step is shortened to end there).  If info is not
  * NULL it receives the record of the last step. */
 int qdeq_rk43_step(qdeq_ctx *ctx, int nsteps, qdeq_step_info *info);
+/* Ensemble stepping (no reference counterpart): advances nsteps accepted
+ * steps of each of n independent contexts (any grids and settings, each
+ * after qdeq_rk43_setup, each on its own stream) from one host thread.  The
+ * driver is a resumable state machine: it never waits on one member while
+ * another can launch work, so small grids' latency-bound kernels overlap.
+ * Each member runs the same launches in the same order as qdeq_rk43_step,
+ * so its results are bit-identical to running it alone.  status[i] (may be
+ * NULL) receives each member's status; the return value is QDEQ_OK or the
+ * last failure. */
+int qdeq_ensemble_step(qdeq_ctx **ctxs, int n, int nsteps, int *status);
+/* ---- output records and event catalogue ---- */
+
+/* The reference's fault output (odeSolver.cpp:1442-1457, 1638-1652): writes
+ * dir/output_fault_0.bin now (t, V, Slip, Ts, psi, U, Tn, a, b, dt after the
+ * t = 0 stage, which runs first if needed) and, after every `every`-th
+ * accepted step (it_save_jump, SetParams.cpp:27), dir/output_fault_<k>.bin
+ * with t, V, Slip, Ts, psi, dt (FP64, nz per array, native byte order).
+ * Records are copied into pinned host slots on the context's stream and
+ * written once complete; qdeq_output_close (and qdeq_destroy) flush them. */
+int qdeq_output_open(qdeq_ctx *ctx, const char *dir, int every);
+int qdeq_output_close(qdeq_ctx *ctx);
+
+/* Event catalogue (no reference counterpart): after every accepted step a
+ * kernel tracks max V; an event opens at the first step with max V above
+ * v_threshold and closes at the first step back at or below it.  Times are
+ * those after the step; nodes are fault rows 1..nk; slip_max is the largest
+ * slip accumulated on any node since the onset; an open event has
+ * t_end = -1, it_end = -1. */
+typedef struct qdeq_event {
+  double t_onset, t_end, t_peak, v_peak, slip_max;
+  int it_onset, it_end, node_onset, node_peak;
+} qdeq_event;
+int qdeq_events_enable(qdeq_ctx *ctx, double v_threshold, int capacity);
+/* Copies up to max entries; *n = events recorded (may exceed capacity). */
+int qdeq_events_get(qdeq_ctx *ctx, qdeq_event *out, int max, int *n);
+
 /* Copies the current state to host arrays (any may be NULL): U (nx*nz, with
  * ghosts as ode_solve leaves them), slip, psi, V, dpsi, Ts (nz). */
 int qdeq_rk43_get_state(qdeq_ctx *ctx, qdeq_step_info *info, double *U,

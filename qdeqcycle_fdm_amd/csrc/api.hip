@@ -93,6 +93,9 @@ static void free_ctx(qdeq_ctx *c) {
   if (c->ev_t0) (void)hipEventDestroy(c->ev_t0);
   if (c->ev_t1) (void)hipEventDestroy(c->ev_t1);
   guess_free_ctx(c);
+  (void)qdeq_output_close(c);
+  events_free(c);
+  if (c->rk.ev) (void)hipEventDestroy(c->rk.ev);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -560,18 +563,6 @@ static int check_flags(qdeq_ctx *c) {
   return QDEQ_OK;
 }
 
-// One stage: boundary data, elliptic solve, fault update.
-static int run_stage(qdeq_ctx *c, int s) {
-  launch_stage(c, s, c->t, c->dt);
-  int st = guess_before_solve(c);
-  if (st != QDEQ_OK) return st;
-  st = mg_solve(c, nullptr, nullptr, nullptr);
-  if (st != QDEQ_OK) return st;
-  if ((st = guess_after_solve(c)) != QDEQ_OK) return st;
-  launch_fault_update(c, s);
-  return QDEQ_OK;
-}
-
 int qdeq_rk43_setup(qdeq_ctx *c, const double *a, const double *b,
                     const double *eta, const double *tau_n0,
                     const double *U0) {
@@ -612,75 +603,256 @@ int qdeq_rk43_setup(qdeq_ctx *c, const double *a, const double *b,
   c->totErr = 0.0;
   c->vcycles = c->solves = c->ir_sweeps = 0;
   c->rk_ready = false;
+  c->rk.phase = 0;  // idle
+  c->mg.phase = 0;
   guess_reset(c);
   return QDEQ_OK;
 }
 
-static int rk43_first_stage(qdeq_ctx *c) {
-  // odeSolver.cpp:1428-1437: stage 1 at t = 0 from y_n.
-  int st = run_stage(c, 1);
-  if (st != QDEQ_OK) return st;
+// ---- resumable RK43 driver (odeSolver.cpp:1428-1633) ----
+//
+// The step loop of the reference as a state machine: every wait (a solve's
+// norms, the embedded error and the NaN flag after the combination) is a
+// phase boundary, so one host thread can advance several contexts, each on
+// its own stream, without blocking on any of them (qdeq_ensemble_step).
+// qdeq_rk43_step drives the same machine with blocking waits, so a context
+// launches the same work in the same order either way.
+enum {
+  RK_IDLE = 0,
+  RK_FIRST,          // stage 1 at t = 0 from y_n (odeSolver.cpp:1428-1437)
+  RK_STEP_BEGIN,
+  RK_ATTEMPT_BEGIN,
+  RK_STAGE_BEGIN,
+  RK_SOLVING,
+  RK_WAIT_COMBINE,
+  RK_WAIT_FIRST,
+  RK_STEP_FINISH,
+  RK_WAIT_END,
+  RK_DONE
+};
+
+// Host readback of n ints of flags (and the embedded error) + an event
+static int rk_readback(qdeq_ctx *c, bool toterr) {
+  RkMachine &r = c->rk;
+  if (!r.ev) QDEQ_HIP(c, hipEventCreateWithFlags(&r.ev, hipEventDisableTiming));
+  QDEQ_HIP(c, hipGetLastError());
+  if (toterr)
+    QDEQ_HIP(c, hipMemcpyAsync(c->scal_host + SC_TOTERR, c->scal + SC_TOTERR,
+                               sizeof(double), hipMemcpyDeviceToHost, c->stream));
   QDEQ_HIP(c, hipMemcpyAsync(c->flags_host, c->flags, sizeof(int),
                              hipMemcpyDeviceToHost, c->stream));
-  QDEQ_HIP(c, hipStreamSynchronize(c->stream));
-  if ((st = check_flags(c))) return st;
-  c->rk_ready = true;
+  QDEQ_HIP(c, hipEventRecord(r.ev, c->stream));
+  return QDEQ_OK;
+}
+
+// *ready = 1 when the last readback has landed.
+static int rk_readback_ready(qdeq_ctx *c, bool block, int *ready) {
+  RkMachine &r = c->rk;
+  if (block) {
+    QDEQ_HIP(c, hipEventSynchronize(r.ev));
+    *ready = 1;
+    return QDEQ_OK;
+  }
+  const hipError_t e = hipEventQuery(r.ev);
+  if (e == hipErrorNotReady) {
+    *ready = 0;
+    return QDEQ_OK;
+  }
+  QDEQ_HIP(c, e);
+  *ready = 1;
+  return QDEQ_OK;
+}
+
+// Advances the machine as far as it can; *done = 1 when the requested steps
+// (and the final flag check) are complete.  With block, every wait blocks.
+static int rk_advance(qdeq_ctx *c, bool block, int *done) {
+  RkMachine &r = c->rk;
+  const qdeq_config &p = c->cfg;
+  *done = 0;
+  int st, ready;
+  for (;;) {
+    switch (r.phase) {
+      case RK_FIRST:
+        r.s = 1;
+        r.final_stage = false;
+        r.phase = RK_STAGE_BEGIN;
+        break;
+      case RK_STEP_BEGIN:
+        // while (it < maxNumSteps && t_elapsed < tmax) (odeSolver.cpp:1462)
+        if (r.steps_left <= 0 || !(c->t < p.tmax)) {
+          if ((st = rk_readback(c, false))) return st;
+          r.phase = RK_WAIT_END;
+          break;
+        }
+        c->it = c->it + 1;
+        r.attempts = 0;
+        r.phase = RK_ATTEMPT_BEGIN;
+        break;
+      case RK_ATTEMPT_BEGIN:  // odeSolver.cpp:1466-1604
+        r.attempts = r.attempts + 1;
+        if (c->t + c->dt > p.tmax) c->dt = p.tmax - c->t;
+        launch_zero_stage_rates(c);
+        r.s = 2;
+        r.final_stage = false;
+        r.phase = RK_STAGE_BEGIN;
+        break;
+      case RK_STAGE_BEGIN:  // boundary data, then the elliptic solve
+        launch_stage(c, r.s, c->t, c->dt);
+        if ((st = guess_before_solve(c))) return st;
+        if ((st = mg_begin(c))) return st;
+        r.phase = RK_SOLVING;
+        break;
+      case RK_SOLVING: {
+        int sdone = 0;
+        if ((st = mg_poll(c, block, &sdone))) return st;
+        if (!sdone) return QDEQ_OK;
+        if ((st = guess_after_solve(c))) return st;
+        launch_fault_update(c, r.s);
+        if (c->rk_ready && !r.final_stage) {
+          if (r.s < 6) {
+            r.s++;
+            r.phase = RK_STAGE_BEGIN;
+          } else {
+            launch_combine(c, c->dt);
+            if ((st = rk_readback(c, true))) return st;
+            r.phase = RK_WAIT_COMBINE;
+          }
+        } else if (!c->rk_ready) {
+          if ((st = rk_readback(c, false))) return st;
+          r.phase = RK_WAIT_FIRST;
+        } else {
+          r.phase = RK_STEP_FINISH;
+        }
+        break;
+      }
+      case RK_WAIT_FIRST:
+        if ((st = rk_readback_ready(c, block, &ready))) return st;
+        if (!ready) return QDEQ_OK;
+        if ((st = check_flags(c))) return st;
+        c->rk_ready = true;
+        r.phase = RK_STEP_BEGIN;
+        break;
+      case RK_WAIT_COMBINE: {
+        if ((st = rk_readback_ready(c, block, &ready))) return st;
+        if (!ready) return QDEQ_OK;
+        if ((st = check_flags(c))) return st;
+        c->totErr = c->scal_host[SC_TOTERR];
+        bool accept = c->totErr < p.totTol;
+        if (!accept) {
+          c->dt = qdeq_compute_step_size(c->it, c->dt, p.mindt, p.maxdt, c->totErr,
+                                         p.totTol, c->errA);
+          accept = p.mindt == c->dt || r.attempts >= 100;
+        }
+        if (!accept) {
+          r.phase = RK_ATTEMPT_BEGIN;
+          break;
+        }
+        c->attempts = r.attempts;
+        c->t = c->t + c->dt;  // odeSolver.cpp:1605-1619
+        launch_commit(c);
+        r.s = 1;
+        r.final_stage = true;
+        r.phase = RK_STAGE_BEGIN;
+        break;
+      }
+      case RK_STEP_FINISH: {
+        double newDeltaT;  // odeSolver.cpp:1621-1632
+        if (c->totErr != 0.0)
+          newDeltaT = qdeq_compute_step_size(c->it, c->dt, p.mindt, p.maxdt,
+                                             c->totErr, p.totTol, c->errA);
+        else
+          newDeltaT = c->dt;
+        c->errA[1] = c->errA[0];
+        c->errA[0] = c->totErr;
+        c->dt = newDeltaT;
+        // per-step record (odeSolver.cpp:1634-1652): events, output file
+        if (c->ev_on) events_after_step(c);
+        if ((st = output_after_step(c))) return st;
+        r.steps_left--;
+        r.phase = RK_STEP_BEGIN;
+        break;
+      }
+      case RK_WAIT_END:
+        if ((st = rk_readback_ready(c, block, &ready))) return st;
+        if (!ready) return QDEQ_OK;
+        if ((st = check_flags(c))) return st;
+        r.phase = RK_DONE;
+        *done = 1;
+        return QDEQ_OK;
+      case RK_DONE:
+        *done = 1;
+        return QDEQ_OK;
+      default:
+        return set_error(c, QDEQ_ERR_STATE, "RK43 driver has no steps requested");
+    }
+  }
+}
+
+// Requests nsteps more accepted steps (and the t = 0 stage first if needed).
+static int rk_request(qdeq_ctx *c, int nsteps) {
+  if (c->mg.phase != 0 ||
+      (c->rk.phase != RK_IDLE && c->rk.phase != RK_DONE))
+    return set_error(c, QDEQ_ERR_STATE, "RK43 driver is busy");
+  c->rk.steps_left = nsteps;
+  c->rk.phase = c->rk_ready ? RK_STEP_BEGIN : RK_FIRST;
   return QDEQ_OK;
 }
 
 int qdeq_rk43_step(qdeq_ctx *c, int nsteps, qdeq_step_info *info) {
-  if (!c) return set_error(c, QDEQ_ERR_ARG, "null argument");
-  int st;
-  if (!c->rk_ready) {
-    if ((st = rk43_first_stage(c))) return st;
-  }
-  const qdeq_config &p = c->cfg;
-  // while (t < tmax) of the reference's driver loop (odeSolver.cpp:1462)
-  for (int n = 0; n < nsteps && c->t < p.tmax; n++) {
-    c->it = c->it + 1;
-    int attemptCount = 0;
-    while (attemptCount < 100) {  // odeSolver.cpp:1466-1604
-      attemptCount = attemptCount + 1;
-      if (c->t + c->dt > p.tmax) c->dt = p.tmax - c->t;
-      launch_zero_stage_rates(c);
-      for (int s = 2; s <= 6; s++)
-        if ((st = run_stage(c, s))) return st;
-      launch_combine(c, c->dt);
-      QDEQ_HIP(c, hipGetLastError());
-      QDEQ_HIP(c, hipMemcpyAsync(c->scal_host + SC_TOTERR, c->scal + SC_TOTERR,
-                                 sizeof(double), hipMemcpyDeviceToHost,
-                                 c->stream));
-      QDEQ_HIP(c, hipMemcpyAsync(c->flags_host, c->flags, sizeof(int),
-                                 hipMemcpyDeviceToHost, c->stream));
-      QDEQ_HIP(c, hipStreamSynchronize(c->stream));
-      if ((st = check_flags(c))) return st;
-      c->totErr = c->scal_host[SC_TOTERR];
-      if (c->totErr < p.totTol) break;
-      c->dt = qdeq_compute_step_size(c->it, c->dt, p.mindt, p.maxdt, c->totErr,
-                                     p.totTol, c->errA);
-      if (p.mindt == c->dt) break;
+  if (!c || nsteps < 0) return set_error(c, QDEQ_ERR_ARG, "bad argument");
+  int st = rk_request(c, nsteps);
+  if (st != QDEQ_OK) return st;
+  int done = 0;
+  while (!done) {
+    if ((st = rk_advance(c, true, &done))) {
+      c->rk.phase = RK_IDLE;
+      c->mg.phase = 0;
+      return st;
     }
-    c->attempts = attemptCount;
-    c->t = c->t + c->dt;  // odeSolver.cpp:1605-1619
-    launch_commit(c);
-    if ((st = run_stage(c, 1))) return st;
-    double newDeltaT;  // odeSolver.cpp:1621-1632
-    if (c->totErr != 0.0)
-      newDeltaT = qdeq_compute_step_size(c->it, c->dt, p.mindt, p.maxdt,
-                                         c->totErr, p.totTol, c->errA);
-    else
-      newDeltaT = c->dt;
-    c->errA[1] = c->errA[0];
-    c->errA[0] = c->totErr;
-    c->dt = newDeltaT;
   }
-  QDEQ_HIP(c, hipMemcpyAsync(c->flags_host, c->flags, sizeof(int),
-                             hipMemcpyDeviceToHost, c->stream));
-  QDEQ_HIP(c, hipStreamSynchronize(c->stream));
-  if ((st = check_flags(c))) return st;
+  c->rk.phase = RK_IDLE;
   if (info) return qdeq_rk43_get_state(c, info, nullptr, nullptr, nullptr,
                                        nullptr, nullptr, nullptr);
   return QDEQ_OK;
+}
+
+int qdeq_ensemble_step(qdeq_ctx **ctxs, int n, int nsteps, int *status) {
+  if (!ctxs || n < 1 || nsteps < 0)
+    return set_error(nullptr, QDEQ_ERR_ARG, "bad argument");
+  std::vector<int> done((size_t)n, 0);
+  int st_all = QDEQ_OK;
+  for (int i = 0; i < n; i++) {
+    int st = ctxs[i] ? rk_request(ctxs[i], nsteps) : QDEQ_ERR_ARG;
+    if (status) status[i] = st;
+    if (st != QDEQ_OK) {
+      done[(size_t)i] = 1;
+      st_all = st;
+    }
+  }
+  // round robin: every member advances as far as it can without waiting
+  int left = 0;
+  for (int i = 0; i < n; i++) left += done[(size_t)i] ? 0 : 1;
+  while (left > 0) {
+    for (int i = 0; i < n; i++) {
+      if (done[(size_t)i]) continue;
+      qdeq_ctx *c = ctxs[i];
+      int d = 0;
+      const int st = rk_advance(c, false, &d);
+      if (st != QDEQ_OK || d) {
+        if (st != QDEQ_OK) {
+          // let the member's queued work drain before it is reused
+          (void)hipStreamSynchronize(c->stream);
+          c->mg.phase = 0;
+          st_all = st;
+        }
+        c->rk.phase = RK_IDLE;
+        if (status) status[i] = st;
+        done[(size_t)i] = 1;
+        left--;
+      }
+    }
+  }
+  return st_all;
 }
 
 int qdeq_rk43_get_state(qdeq_ctx *c, qdeq_step_info *info, double *U,

@@ -475,58 +475,144 @@ int mbox_wait(qdeq_ctx *c, int slot) {
 
 // multigrid (odeSolver.cpp:652-720) on the device-resident level-0 u whose
 // Dirichlet columns are already set.  Iterates while ||r||_2 > threshold.
-int mg_solve(qdeq_ctx *c, int *vcycles, double *res0, double *res) {
-  const bool ir = c->cfg.precision == QDEQ_PREC_FP32_IR;
+// Non-blocking check of a mailbox slot: *ready = 1 once written.  Launch
+// failures surface through an occasional hipStreamQuery, and a slot still
+// empty 200 ms after the stream went idle is an error.
+static int mbox_check(qdeq_ctx *c, int slot, bool block, int *ready) {
+  if (block) {
+    const int st = mbox_wait(c, slot);
+    *ready = st == QDEQ_OK;
+    return st;
+  }
+  volatile double *p = reinterpret_cast<volatile double *>(c->mbox_host) + slot;
+  if (*p != kMboxEmpty) {
+    std::atomic_thread_fence(std::memory_order_acquire);
+    c->mb_idle = false;
+    *ready = 1;
+    return QDEQ_OK;
+  }
+  *ready = 0;
+  if ((++c->mb_polls & 255) != 0) return QDEQ_OK;
+  const hipError_t e = hipStreamQuery(c->stream);
+  if (e != hipSuccess && e != hipErrorNotReady)
+    return set_error(c, QDEQ_ERR_HIP,
+                     std::string("waiting for a norm: ") + hipGetErrorString(e));
+  if (e == hipSuccess) {
+    const auto now = std::chrono::steady_clock::now();
+    if (!c->mb_idle) {
+      c->mb_idle = true;
+      c->mb_idle_since = now;
+    } else if (now - c->mb_idle_since > std::chrono::milliseconds(200)) {
+      c->mb_idle = false;
+      return set_error(c, QDEQ_ERR_STATE, "norm mailbox was not written");
+    }
+  }
+  return QDEQ_OK;
+}
+
+// multigrid (odeSolver.cpp:652-720) on the device-resident level-0 u whose
+// Dirichlet columns are already set, as a resumable machine: mg_begin
+// enqueues the initial residual norms; mg_poll consumes each norm as it
+// arrives and enqueues the next V-cycle while ||r||_2 > threshold, without
+// blocking unless asked to.  A solve runs the same launches in the same order
+// whether it is polled or waited for.
+int mg_begin(qdeq_ctx *c) {
   int st = ensure_graph(c);
   if (st != QDEQ_OK) return st;
+  MgSolve &m = c->mg;
+  m.ir = c->cfg.precision == QDEQ_PREC_FP32_IR;
   // a predicted start (learned guess) was enqueued before this solve: its
   // candidate sits in the level-0 r buffer, its residual norm in
   // scal[SC_NORM_PRED] (and the mailbox slot, armed before the prediction)
-  const bool cand = c->cand_pending;
+  m.cand = c->cand_pending;
   c->cand_pending = false;
+  m.it = 0;
+  m.sample = false;
   mbox_arm(c, SC_NORM);
   mbox_arm(c, SC_NORM0);
   if (c->cfg.tol_mode == QDEQ_TOL_REL) launch_bdres(c);
-  launch_resnorm(c, ir);
+  launch_resnorm(c, m.ir);
   QDEQ_HIP(c, hipGetLastError());
-  if ((st = mbox_wait(c, SC_NORM))) return st;
-  if (c->cfg.tol_mode == QDEQ_TOL_REL && (st = mbox_wait(c, SC_NORM0))) return st;
-  if (cand && (st = mbox_wait(c, SC_NORM_PRED))) return st;
-  for (int i : {SC_NORM, SC_NORM0, SC_NORM_PRED}) c->scal_host[i] = c->mbox_host[i];
-  double delta = c->scal_host[SC_NORM];
-  if (cand) {
-    // used only if its residual is finite and below the previous solution's
-    const double pn = c->scal_host[SC_NORM_PRED];
-    if (std::isfinite(pn) && pn < delta) {
-      const LevelDims &d = c->lv[0];
-      QDEQ_HIP(c, hipMemcpyAsync(c->d[0].u, c->d[0].r,
-                                 (size_t)d.N * d.P * sizeof(double),
-                                 hipMemcpyDeviceToDevice, c->stream));
-      if (ir) launch_resnorm(c, true);  // FP32 residual of the new start
-      delta = pn;
-      c->g_accepted++;
+  m.phase = 1;
+  return QDEQ_OK;
+}
+
+static int mg_finish(qdeq_ctx *c, int st) {
+  MgSolve &m = c->mg;
+  c->vcycles += m.it;
+  if (m.ir) c->ir_sweeps += m.it;
+  c->solves += 1;
+  m.phase = 0;
+  return st;
+}
+
+int mg_poll(qdeq_ctx *c, bool block, int *done) {
+  MgSolve &m = c->mg;
+  *done = 0;
+  int st, ready;
+  for (;;) {
+    if (m.phase == 1) {
+      if ((st = mbox_check(c, SC_NORM, block, &ready)) || !ready) break;
+      if (c->cfg.tol_mode == QDEQ_TOL_REL &&
+          ((st = mbox_check(c, SC_NORM0, block, &ready)) || !ready))
+        break;
+      if (m.cand && ((st = mbox_check(c, SC_NORM_PRED, block, &ready)) || !ready))
+        break;
+      for (int i : {SC_NORM, SC_NORM0, SC_NORM_PRED}) c->scal_host[i] = c->mbox_host[i];
+      m.delta = c->scal_host[SC_NORM];
+      if (m.cand) {
+        // used only if its residual is finite and below the previous solution's
+        const double pn = c->scal_host[SC_NORM_PRED];
+        if (std::isfinite(pn) && pn < m.delta) {
+          const LevelDims &d = c->lv[0];
+          QDEQ_HIP(c, hipMemcpyAsync(c->d[0].u, c->d[0].r,
+                                     (size_t)d.N * d.P * sizeof(double),
+                                     hipMemcpyDeviceToDevice, c->stream));
+          if (m.ir) launch_resnorm(c, true);  // FP32 residual of the new start
+          m.delta = pn;
+          c->g_accepted++;
+        }
+      }
+      m.r0 = c->cfg.tol_mode == QDEQ_TOL_REL ? c->scal_host[SC_NORM0] : m.delta;
+      m.thr = c->cfg.tol_mode == QDEQ_TOL_REL ? c->cfg.tol * m.r0 : c->cfg.tol;
+    } else if (m.phase == 2) {
+      if ((st = mbox_check(c, SC_NORM, block, &ready)) || !ready) break;
+      if (m.sample) {
+        float ms = 0.0f;
+        QDEQ_HIP(c, hipEventSynchronize(c->ev_t1));
+        if (hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1) == hipSuccess) {
+          c->timed_ms += ms;
+          c->timed_calls += 1;
+        }
+      }
+      m.delta = c->scal_host[SC_NORM] = c->mbox_host[SC_NORM];
+      m.it++;
+      if (!std::isfinite(m.delta)) {
+        *done = 1;
+        return mg_finish(c, set_error(c, QDEQ_ERR_MG_NOT_CONVERGED,
+                                      "multigrid: residual is not finite"));
+      }
+    } else {
+      return set_error(c, QDEQ_ERR_STATE, "multigrid: no solve in progress");
     }
-  }
-  const double r0 =
-      c->cfg.tol_mode == QDEQ_TOL_REL ? c->scal_host[SC_NORM0] : delta;
-  const double thr =
-      c->cfg.tol_mode == QDEQ_TOL_REL ? c->cfg.tol * r0 : c->cfg.tol;
-  int it = 0;
-  while (delta > thr) {
-    if (it >= c->cfg.max_vcycles) {
-      st = set_error(c, QDEQ_ERR_MG_NOT_CONVERGED,
-                     "multigrid: V-cycle cap reached (residual " +
-                         std::to_string(delta) + " > " + std::to_string(thr) +
-                         ")");
-      break;
+    // decide: converged, capped, or one more V-cycle
+    if (!(m.delta > m.thr)) {
+      *done = 1;
+      return mg_finish(c, QDEQ_OK);
+    }
+    if (m.it >= c->cfg.max_vcycles) {
+      *done = 1;
+      return mg_finish(c, set_error(c, QDEQ_ERR_MG_NOT_CONVERGED,
+                                    "multigrid: V-cycle cap reached (residual " +
+                                        std::to_string(m.delta) + " > " +
+                                        std::to_string(m.thr) + ")"));
     }
     mbox_arm(c, SC_NORM);
     // kernel timing samples every kTimingEvery-th V-cycle: that one is
     // launched directly (same kernels and arguments as the graph) with the
     // events recorded around the timed kernel
-    const bool sample = c->timing != 0 && use_tiled(c) &&
-                        (c->timed_seq++ % kTimingEvery) == 0;
-    if (sample) {
+    m.sample = c->timing != 0 && use_tiled(c) && (c->timed_seq++ % kTimingEvery) == 0;
+    if (m.sample) {
       c->emit_events = true;
       launch_iteration(c);
       c->emit_events = false;
@@ -534,29 +620,30 @@ int mg_solve(qdeq_ctx *c, int *vcycles, double *res0, double *res) {
     } else {
       QDEQ_HIP(c, hipGraphLaunch(c->g_cycle, c->stream));
     }
-    if ((st = mbox_wait(c, SC_NORM))) return st;
-    if (sample) {
-      float ms = 0.0f;
-      QDEQ_HIP(c, hipEventSynchronize(c->ev_t1));
-      if (hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1) == hipSuccess) {
-        c->timed_ms += ms;
-        c->timed_calls += 1;
-      }
-    }
-    delta = c->scal_host[SC_NORM] = c->mbox_host[SC_NORM];
-    it++;
-    if (!std::isfinite(delta)) {
-      st = set_error(c, QDEQ_ERR_MG_NOT_CONVERGED,
-                     "multigrid: residual is not finite");
-      break;
+    m.phase = 2;
+    if (!block) {
+      // the norm can only be ready once the V-cycle ran: return now
+      return QDEQ_OK;
     }
   }
-  c->vcycles += it;
-  if (ir) c->ir_sweeps += it;
-  c->solves += 1;
-  if (vcycles) *vcycles = it;
-  if (res0) *res0 = r0;
-  if (res) *res = delta;
+  if (st != QDEQ_OK) {
+    *done = 1;
+    return mg_finish(c, st);
+  }
+  return QDEQ_OK;
+}
+
+int mg_solve(qdeq_ctx *c, int *vcycles, double *res0, double *res) {
+  int st = mg_begin(c);
+  if (st != QDEQ_OK) return st;
+  int done = 0;
+  while (!done) {
+    st = mg_poll(c, true, &done);
+    if (st != QDEQ_OK) break;
+  }
+  if (vcycles) *vcycles = c->mg.it;
+  if (res0) *res0 = c->mg.r0;
+  if (res) *res = c->mg.delta;
   return st;
 }
 

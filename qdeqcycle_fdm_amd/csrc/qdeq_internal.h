@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <string>
 #include <vector>
 
@@ -59,6 +60,26 @@ enum {
   SC_TSMAX = 4,
   SC_NORM_PRED = 5,  // residual norm of the learned-guess candidate
   SC_COUNT = 8
+};
+
+struct OutputRing;  // output.hip
+
+// State of a solve in progress (mg_begin / mg_poll).
+struct MgSolve {
+  int phase = 0;        // 0 idle, 1 initial norms pending, 2 V-cycle pending
+  bool ir = false, cand = false, sample = false;
+  int it = 0;
+  double delta = 0, r0 = 0, thr = 0;
+};
+
+// State of the resumable RK43 driver (api.hip).
+struct RkMachine {
+  int phase = 0;
+  int s = 0;               // stage being solved
+  int attempts = 0;        // attempts of the current step
+  long long steps_left = 0;
+  bool final_stage = false;  // stage 1 after an accepted step
+  hipEvent_t ev = nullptr;   // completion of the last host readback
 };
 
 }  // namespace qdeq
@@ -124,6 +145,20 @@ struct qdeq_ctx {
   double *gprev = nullptr, *gbefore = nullptr;
   double *gs = nullptr;  // [0] max|ds|, [1] remote change, [2] test factor
   bool gprev_valid = false, cand_pending = false, harvest_pending = false;
+  // resumable solve and RK43 driver; mailbox polling state
+  qdeq::MgSolve mg;
+  qdeq::RkMachine rk;
+  long long mb_polls = 0;
+  // output records and event catalogue (output.hip)
+  qdeq::OutputRing *out = nullptr;
+  int *ev_state = nullptr;
+  void *ev_table = nullptr;
+  double *ev_slip0 = nullptr;
+  double ev_thr = 0.0;
+  int ev_cap = 0;
+  bool ev_on = false;
+  bool mb_idle = false;
+  std::chrono::steady_clock::time_point mb_idle_since{};
   long long g_predicted = 0, g_accepted = 0, g_harvested = 0, g_seen = 0;
   hipGraphExec_t g_pred = nullptr;
 };
@@ -134,6 +169,8 @@ int set_error(qdeq_ctx *c, int code, const std::string &msg);
 
 // multigrid (mg.hip)
 int mg_solve(qdeq_ctx *c, int *vcycles, double *res0, double *res);
+int mg_begin(qdeq_ctx *c);
+int mg_poll(qdeq_ctx *c, bool block, int *done);
 int mg_build_coefficients(qdeq_ctx *c);
 void mg_destroy_graphs(qdeq_ctx *c);
 bool use_tiled(const qdeq_ctx *c);
@@ -141,6 +178,13 @@ bool use_tiled(const qdeq_ctx *c);
 void launch_resnorm_of(qdeq_ctx *c, const double *u, int off, int slot);
 void mbox_arm(qdeq_ctx *c, int slot);
 int mbox_wait(qdeq_ctx *c, int slot);
+
+// output records and events (output.hip); output_after_step writes the
+// periodic record when one is due
+int output_after_step(qdeq_ctx *c);
+void output_free(qdeq_ctx *c);
+void events_after_step(qdeq_ctx *c);
+void events_free(qdeq_ctx *c);
 
 // learned initial guess (guess.hip)
 int guess_before_solve(qdeq_ctx *c);

@@ -40,7 +40,9 @@ EXPORTS = (
     "qdeq_rk43_get_state", "qdeq_set_tiles", "qdeq_uses_tiled_cycle",
     "qdeq_set_kernel_timing", "qdeq_get_kernel_timing", "qdeq_copy_probe",
     "qdeq_device_synchronize", "qdeq_autotune_tiles", "qdeq_get_tile_shapes",
-    "qdeq_kernel_breakdown", "qdeq_set_tile_shapes",
+    "qdeq_kernel_breakdown", "qdeq_set_tile_shapes", "qdeq_ensemble_step",
+    "qdeq_output_open", "qdeq_output_close", "qdeq_events_enable",
+    "qdeq_events_get",
     # include/qdeq_guess.h
     "qdeq_guess_default_config", "qdeq_train_default_config",
     "qdeq_guess_create", "qdeq_guess_destroy", "qdeq_guess_last_error",
@@ -84,6 +86,19 @@ class StepInfo(ctypes.Structure):
         ("Ts_max", ctypes.c_double), ("it", ctypes.c_int),
         ("attempts", ctypes.c_int), ("vcycles", ctypes.c_longlong),
         ("solves", ctypes.c_longlong), ("ir_sweeps", ctypes.c_longlong),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Event(ctypes.Structure):
+    _fields_ = [
+        ("t_onset", ctypes.c_double), ("t_end", ctypes.c_double),
+        ("t_peak", ctypes.c_double), ("v_peak", ctypes.c_double),
+        ("slip_max", ctypes.c_double), ("it_onset", ctypes.c_int),
+        ("it_end", ctypes.c_int), ("node_onset", ctypes.c_int),
+        ("node_peak", ctypes.c_int),
     ]
 
     def as_dict(self) -> dict:
@@ -152,6 +167,11 @@ def load(path: str = LIB_PATH):
     L.qdeq_get_tile_shapes.argtypes = [_vp, _ip, _ip]
     L.qdeq_kernel_breakdown.argtypes = [_vp, ctypes.c_int, _ip, _dp]
     L.qdeq_set_tile_shapes.argtypes = [_vp, ctypes.c_int, _ip]
+    L.qdeq_ensemble_step.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, _ip]
+    L.qdeq_output_open.argtypes = [_vp, ctypes.c_char_p, ctypes.c_int]
+    L.qdeq_output_close.argtypes = [_vp]
+    L.qdeq_events_enable.argtypes = [_vp, ctypes.c_double, ctypes.c_int]
+    L.qdeq_events_get.argtypes = [_vp, ctypes.POINTER(Event), ctypes.c_int, _ip]
     L.qdeq_set_kernel_timing.argtypes = [_vp, ctypes.c_int]
     L.qdeq_get_kernel_timing.argtypes = [_vp, _dp, ctypes.POINTER(ctypes.c_longlong)]
     L.qdeq_copy_probe.argtypes = [_vp, ctypes.c_longlong, ctypes.c_int, _dp]
@@ -328,6 +348,24 @@ class Engine:
         buf = (ctypes.c_int * max(len(flat), 1))(*flat)
         self._check(load().qdeq_set_tile_shapes(self.h, len(shapes), buf))
 
+    def output_open(self, directory: str, every: int = 2):
+        """Reference fault output records (output_fault_<k>.bin) in directory."""
+        self._check(load().qdeq_output_open(self.h, directory.encode(), every))
+
+    def output_close(self):
+        self._check(load().qdeq_output_close(self.h))
+
+    def events_enable(self, v_threshold: float = 1e-3, capacity: int = 4096):
+        self._cap = capacity
+        self._check(load().qdeq_events_enable(self.h, v_threshold, capacity))
+
+    def events(self) -> list:
+        cap = getattr(self, "_cap", 4096)
+        buf = (Event * cap)()
+        n = ctypes.c_int(0)
+        self._check(load().qdeq_events_get(self.h, buf, cap, ctypes.byref(n)))
+        return [buf[i].as_dict() for i in range(min(n.value, cap))]
+
     def tile_shapes(self):
         """[(down (tx, ty, rows), up (tx, ty, rows)) per tile level]."""
         n = ctypes.c_int(0)
@@ -437,6 +475,23 @@ class Engine:
         out["U"] = U
         out["info"] = info.as_dict()
         return out
+
+
+def ensemble_step(engines, nsteps: int):
+    """Advances nsteps accepted steps of every engine together
+    (qdeq_ensemble_step); returns the per-engine status list and raises if
+    any failed."""
+    n = len(engines)
+    hs = (_vp * n)(*[e.h for e in engines])
+    st = (ctypes.c_int * n)()
+    rc = load().qdeq_ensemble_step(hs, n, nsteps, st)
+    status = list(st)
+    if rc != QDEQ_OK:
+        bad = [i for i, s in enumerate(status) if s != QDEQ_OK]
+        msgs = "; ".join(f"member {i}: {load().qdeq_last_error(engines[i].h).decode()}"
+                         for i in bad)
+        raise QdeqError(rc, msgs or load().qdeq_global_error().decode())
+    return status
 
 
 class Guess:

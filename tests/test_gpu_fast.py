@@ -193,3 +193,26 @@ def test_pinned_tile_shapes_round_trip_and_validation():
         eng.ode_solve(V)
     V2, _ = _gpu_solve(pb, me, U, True)
     np.testing.assert_array_equal(V, V2)
+
+
+def test_kernel_timing_does_not_change_results():
+    """Timed V-cycles are launched directly instead of from the graph: the
+    run must be bit-identical, and the timed kernel must have been timed."""
+    pb = config.make_bp1_problem(257, basin=True, levels=5,
+                                 tol_mode=config.TOL_REL, tol=1e-10)
+    me = oracle.mu_equi(pb.mu)
+    out = []
+    for which in (lib.TIME_OFF, lib.TIME_UP0, lib.TIME_TAIL):
+        with lib.Engine(pb, me, **FAST) as eng:
+            eng.set_tiles(True, 64, 16)
+            eng.rk43_setup()
+            eng.set_kernel_timing(which)
+            info = eng.rk43_step(30)
+            ms, calls = eng.kernel_timing()
+            if which != lib.TIME_OFF:
+                assert calls > 0 and ms > 0
+            out.append((info, eng.state()))
+    for info, st in out[1:]:
+        assert info["t"] == out[0][0]["t"] and info["vcycles"] == out[0][0]["vcycles"]
+        for k in ("slip", "psi", "U"):
+            np.testing.assert_array_equal(st[k], out[0][1][k])

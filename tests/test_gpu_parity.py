@@ -173,6 +173,7 @@ def test_rk43_bp1_129_golden_2000_steps():
     col = {n: i for i, n in enumerate(names)}
     with lib.Engine(pb, oracle.mu_equi(pb.mu)) as eng:
         eng.rk43_setup()
+        eng.events_enable(1e-3)
         snap = 0
         for s in range(series.shape[0]):
             info = eng.rk43_step(1)
@@ -185,6 +186,29 @@ def test_rk43_bp1_129_golden_2000_steps():
                 for key in ("slip", "psi", "V"):
                     np.testing.assert_array_equal(st[key], g[key][snap], err_msg=key)
                 snap += 1
+        events = eng.events()
     onsets = np.flatnonzero((series[1:, col["V_max"]] > 1e-3) &
                             (series[:-1, col["V_max"]] <= 1e-3))
     assert len(onsets) >= 2
+    # the device event catalogue against the fixture's V_max series: step s
+    # (0-based) of the series is accepted step it = s + 1
+    vmax, t = series[:, col["V_max"]], series[:, col["t"]]
+    ref = []
+    s = 0
+    while s < len(vmax):
+        if vmax[s] > 1e-3:
+            e = s
+            while e < len(vmax) and vmax[e] > 1e-3:
+                e += 1
+            pk = s + int(np.argmax(vmax[s:e]))
+            ref.append(dict(it_onset=s + 1, t_onset=t[s], v_peak=vmax[pk],
+                            t_peak=t[pk], it_end=e + 1 if e < len(vmax) else -1,
+                            t_end=t[e] if e < len(vmax) else -1.0))
+            s = e
+        else:
+            s += 1
+    assert len(events) == len(ref) == len(onsets) + (1 if vmax[0] > 1e-3 else 0)
+    for got, exp in zip(events, ref):
+        for key, val in exp.items():
+            assert got[key] == val, (key, got[key], val)
+        assert 1 <= got["node_onset"] <= 129 and got["slip_max"] > 0.0
