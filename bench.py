@@ -212,6 +212,68 @@ def run_learned(pb, me, g, warmup, tmax):
     return eng, out
 
 
+def run_ensemble(members, steps, precision_mode):
+    """Ensemble workload: `members` simulations (seeded draws of grid
+    257/513/1025, Dc, sigma_n, b; basin medium), `steps` accepted steps each,
+    first one after the other (each alone), then all together with
+    qdeq_ensemble_step.  Returns both rates and whether every member's final
+    state is bit-identical between the two runs."""
+    import numpy as np
+    from qdeqcycle_fdm_amd import config, lib
+    levels = {257: 5, 513: 6, 1025: 7}
+    draws = config.ensemble_members(members, seed=0)
+    media, shapes, pbs = {}, {}, []
+    for d in draws:
+        pbs.append(config.make_bp1_problem(d["n"], basin=True, levels=levels[d["n"]],
+                                           Dc=d["Dc"], sigma_n=d["sigma_n"],
+                                           b_seis=d["b_seis"], tol_mode=config.TOL_REL,
+                                           tol=1e-10))
+        if d["n"] not in media:
+            media[d["n"]] = lib.build_mu_equi(pbs[-1].mu)
+
+    def make(i):
+        pb = pbs[i]
+        n = draws[i]["n"]
+        kw = dict(MODES[precision_mode])
+        tiles = kw.pop("tiles")
+        eng = lib.Engine(pb, media[n], **kw)
+        eng.set_tiles(True, *tiles)
+        if n not in shapes:
+            shapes[n] = eng.autotune_tiles(5)
+        else:
+            eng.set_tile_shapes(shapes[n])
+        eng.rk43_setup()
+        return eng
+
+    alone, t_alone = [], 0.0
+    for i in range(members):
+        eng = make(i)
+        sync()
+        t0 = time.perf_counter()
+        eng.rk43_step(steps)
+        sync()
+        t_alone += time.perf_counter() - t0
+        alone.append(eng.state()["slip"])
+        eng.close()
+        if (i + 1) % 8 == 0:
+            log(f"ensemble members alone: {i + 1}/{members} ({t_alone:.1f} s)")
+    engines = [make(i) for i in range(members)]
+    sync()
+    t0 = time.perf_counter()
+    lib.ensemble_step(engines, steps)
+    sync()
+    t_ens = time.perf_counter() - t0
+    same = all(np.array_equal(e.state()["slip"], a) for e, a in zip(engines, alone))
+    for e in engines:
+        e.close()
+    total = members * steps
+    return dict(members=members, steps_each=steps, mode=precision_mode,
+                grids={str(n): sum(1 for d in draws if d["n"] == n) for n in levels},
+                steps_per_s_alone=total / t_alone, steps_per_s_ensemble=total / t_ens,
+                seconds_alone=t_alone, seconds_ensemble=t_ens,
+                bitwise_equal_to_alone=bool(same))
+
+
 def cpu_baseline(pb, me, eng, seconds):
     """Oracle restarted from the engine's state; the engine then runs the
     same steps; returns the CPU rate, the GPU rate on the same steps and
@@ -259,6 +321,11 @@ def main():
     ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--ensemble", type=int, default=0, metavar="M",
+                    help="only run the ensemble workload: M members, "
+                         "--ensemble-steps steps each, alone and together")
+    ap.add_argument("--ensemble-steps", type=int, default=1000)
+    ap.add_argument("--ensemble-mode", default="ir32", choices=("fast64", "ir32"))
     ap.add_argument("--no-autotune", action="store_true",
                     help="default tile shapes instead of the autotuned ones")
     ap.add_argument("--no-learned", action="store_true",
@@ -285,8 +352,21 @@ def main():
               "DESIGN.md). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    if args.ensemble and "GPU_MAX_HW_QUEUES" not in os.environ:
+        # more hardware queues let the members' streams run concurrently
+        os.environ["GPU_MAX_HW_QUEUES"] = "16"
     from qdeqcycle_fdm_amd import config, lib
     lib.load()
+    if args.ensemble:
+        res = run_ensemble(args.ensemble, args.ensemble_steps, args.ensemble_mode)
+        log(f"ensemble: {res['steps_per_s_ensemble']:.1f} steps/s together, "
+            f"{res['steps_per_s_alone']:.1f} one after the other, bitwise equal: "
+            f"{res['bitwise_equal_to_alone']}")
+        print(json.dumps({"metric": "ensemble accepted RK43 steps/s (all members)",
+                          "value": round(res["steps_per_s_ensemble"], 2),
+                          "unit": "steps/s", "n_gpus": 1, "higher_is_better": True,
+                          "data": "synthetic", "ensemble": res}), flush=True)
+        return
     pb = config.make_bp1_problem(1025, basin=True, tol_mode=config.TOL_REL,
                                  tol=1e-10)
     if args.copy_probe:
