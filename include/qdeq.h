@@ -135,7 +135,9 @@ int qdeq_kernel_breakdown(qdeq_ctx *ctx, int reps, int *n, double *us);
  * accumulated duration and the count of timed launches.  qdeq_copy_probe times `reps` launches of a plain
  * device copy of n_doubles doubles (bandwidth and counter calibration). */
 enum { QDEQ_TIME_OFF = 0, QDEQ_TIME_UP0 = 1, QDEQ_TIME_TAIL = 2,
-       QDEQ_TIME_DOWN1 = 3 };
+       QDEQ_TIME_DOWN1 = 3, QDEQ_TIME_FAULT = 4 };
+/* QDEQ_TIME_FAULT times the fault update after each stage solve (traction,
+ * Newton solve for V, state rate), every 8th one, with events around it. */
 int qdeq_set_kernel_timing(qdeq_ctx *ctx, int which);
 /* hipDeviceSynchronize on `device` (a barrier for host-side timing). */
 int qdeq_device_synchronize(int device);

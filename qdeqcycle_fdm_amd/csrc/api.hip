@@ -82,6 +82,7 @@ static void free_ctx(qdeq_ctx *c) {
   (void)hipFree(c->cinv64);
   (void)hipFree(c->cinv32);
   (void)hipFree(c->partial);
+  (void)hipFree(c->gate);
   (void)hipFree(c->scal);
   (void)hipFree(c->flags);
   (void)hipFree(c->stage_U);
@@ -188,6 +189,8 @@ int qdeq_create(const qdeq_config *cfg, const double *mu,
   CREATE_HIP(hipHostMalloc(&c->mbox_host, SC_COUNT * sizeof(double),
                            hipHostMallocMapped | hipHostMallocCoherent));
   CREATE_HIP(hipHostGetDevicePointer((void **)&c->mbox_dev, c->mbox_host, 0));
+  std::memset(c->mbox_host, 0, SC_COUNT * sizeof(double));
+  CREATE_HIP(alloc_zero(&c->gate, 4, c->stream));
   CREATE_HIP(hipHostMalloc(&c->flags_host, 4 * sizeof(int)));
   CREATE_HIP(alloc_zero(&c->stage_U, (size_t)2 * c->nx * c->nz, c->stream));
   for (auto &p : c->stage_v) CREATE_HIP(alloc_zero(&p, (size_t)c->nz, c->stream));
@@ -293,7 +296,7 @@ int qdeq_device_synchronize(int device) {
 }
 
 int qdeq_set_kernel_timing(qdeq_ctx *c, int which) {
-  if (!c || which < 0 || which > QDEQ_TIME_DOWN1)
+  if (!c || which < 0 || which > QDEQ_TIME_FAULT)
     return set_error(c, QDEQ_ERR_ARG, "bad timing selector");
   if (which && !c->ev_t0) {
     QDEQ_HIP(c, hipEventCreate(&c->ev_t0));
@@ -707,7 +710,19 @@ static int rk_advance(qdeq_ctx *c, bool block, int *done) {
         if ((st = mg_poll(c, block, &sdone))) return st;
         if (!sdone) return QDEQ_OK;
         if ((st = guess_after_solve(c))) return st;
-        launch_fault_update(c, r.s);
+        if (c->timing == QDEQ_TIME_FAULT && (c->timed_seq++ % 8) == 0) {
+          QDEQ_HIP(c, hipEventRecord(c->ev_t0, c->stream));
+          launch_fault_update(c, r.s);
+          QDEQ_HIP(c, hipEventRecord(c->ev_t1, c->stream));
+          QDEQ_HIP(c, hipEventSynchronize(c->ev_t1));
+          float ms = 0.0f;
+          if (hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1) == hipSuccess) {
+            c->timed_ms += ms;
+            c->timed_calls += 1;
+          }
+        } else {
+          launch_fault_update(c, r.s);
+        }
         if (c->rk_ready && !r.final_stage) {
           if (r.s < 6) {
             r.s++;

@@ -144,7 +144,8 @@ __global__ __launch_bounds__(NORM_THREADS) void k_resnorm_partial(
     const double *__restrict__ u, const double *__restrict__ cx,
     const double *__restrict__ cz, int N, int K, int P, double DH2,
     double *__restrict__ partial, float *__restrict__ r32,
-    float *__restrict__ e32) {
+    float *__restrict__ e32, const int *__restrict__ gate) {
+  if (gate && gate[0]) return;  // batched solve already converged
   __shared__ double sh[NORM_THREADS];
   const int W = K - 2;
   const long total = (long)N * W;
@@ -209,6 +210,50 @@ __global__ __launch_bounds__(NORM_THREADS) void k_norm_final(
     const double v = sqrt(s);
     out[0] = v;
     if (mb) mb[0] = v;
+  }
+}
+
+// Stop test of one V-cycle of a batched solve.  gate[0]: stop (the solve
+// converged or failed; the remaining cycles of the batch skip), gate[1]:
+// V-cycles done, gate[2]: outcome (0 continue, 1 converged, 2 non-finite
+// residual, 3 V-cycle cap), gate[3]: sequence number of the last cycle seen.
+// The host waits for mb[MB_SEQ] to reach the batch's last sequence number;
+// a skipped cycle only advances it.  The threshold is tol * the boundary-
+// data norm (REL) or tol (ABS), the comparisons those of the host loop.
+__global__ __launch_bounds__(NORM_THREADS) void k_norm_gate(
+    const double *__restrict__ partial, int n, double *__restrict__ scal,
+    double *__restrict__ mb, int *__restrict__ gate, int rel, double tol,
+    int cap) {
+  __shared__ double sh[NORM_THREADS];
+  const bool skip = gate[0] != 0;
+  double acc = 0.0;
+  if (!skip)
+    for (int i = threadIdx.x; i < n; i += NORM_THREADS) acc += partial[i];
+  const double s = block_sum_256(acc, sh);
+  if (threadIdx.x == 0) {
+    const int seq = gate[3] + 1;
+    gate[3] = seq;
+    if (!skip) {
+      const double v = sqrt(s);
+      scal[SC_NORM] = v;
+      mb[SC_NORM] = v;
+      const int cnt = gate[1] + 1;
+      gate[1] = cnt;
+      const double thr = rel ? tol * scal[SC_NORM0] : tol;
+      int code = 0;
+      if (isnan(v) || isinf(v))
+        code = 2;
+      else if (!(v > thr))
+        code = 1;
+      else if (cnt >= cap)
+        code = 3;
+      if (code) gate[0] = 1;
+      gate[2] = code;
+      mb[MB_COUNT] = (double)cnt;
+      mb[MB_CODE] = (double)code;
+    }
+    __threadfence_system();
+    mb[MB_SEQ] = (double)seq;
   }
 }
 
@@ -336,6 +381,19 @@ static void vcycle(qdeq_ctx *c, LevelBufs<T> *B, int level, bool level0_has_f) {
   smooth<T, EXACT>(c, B[level], d, c->cfg.n_post, has_f);
 }
 
+// Stop test of a V-cycle launched inside a batch (gate_ptr set), or the
+// plain norm of the initial check.
+static void launch_norm_final(qdeq_ctx *c, int nparts) {
+  if (c->gate_ptr)
+    hipLaunchKernelGGL(k_norm_gate, dim3(1), dim3(NORM_THREADS), 0, c->stream,
+                       c->partial, nparts, c->scal, c->mbox_dev, c->gate_ptr,
+                       c->cfg.tol_mode == QDEQ_TOL_REL ? 1 : 0, c->cfg.tol,
+                       c->cfg.max_vcycles);
+  else
+    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(NORM_THREADS), 0, c->stream,
+                       c->partial, nparts, c->scal + SC_NORM, c->mbox_dev + SC_NORM);
+}
+
 // Level-0 residual norm into scal[SC_NORM] (and r32/e32 for IR).
 static void launch_resnorm(qdeq_ctx *c, bool store32) {
   const LevelDims &d = c->lv[0];
@@ -344,14 +402,13 @@ static void launch_resnorm(qdeq_ctx *c, bool store32) {
     hipLaunchKernelGGL((k_resnorm_partial<true>), dim3(NORM_BLOCKS),
                        dim3(NORM_THREADS), 0, c->stream, c->d[0].u, c->d[0].cx,
                        c->d[0].cz, d.N, d.K, d.P, DH2, c->partial, c->s[0].f,
-                       c->s[0].u);
+                       c->s[0].u, (const int *)c->gate_ptr);
   else
     hipLaunchKernelGGL((k_resnorm_partial<false>), dim3(NORM_BLOCKS),
                        dim3(NORM_THREADS), 0, c->stream, c->d[0].u, c->d[0].cx,
                        c->d[0].cz, d.N, d.K, d.P, DH2, c->partial,
-                       (float *)nullptr, (float *)nullptr);
-  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(NORM_THREADS), 0, c->stream,
-                     c->partial, NORM_BLOCKS, c->scal + SC_NORM, c->mbox_dev + SC_NORM);
+                       (float *)nullptr, (float *)nullptr, (const int *)c->gate_ptr);
+  launch_norm_final(c, NORM_BLOCKS);
 }
 
 // Residual norm of an arbitrary level-0 displacement u (same operator and
@@ -361,7 +418,7 @@ void launch_resnorm_of(qdeq_ctx *c, const double *u, int off, int slot) {
   hipLaunchKernelGGL((k_resnorm_partial<false>), dim3(NORM_BLOCKS),
                      dim3(NORM_THREADS), 0, c->stream, u, c->d[0].cx, c->d[0].cz,
                      d.N, d.K, d.P, d.DH * d.DH, c->partial + off,
-                     (float *)nullptr, (float *)nullptr);
+                     (float *)nullptr, (float *)nullptr, (const int *)nullptr);
   hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(NORM_THREADS), 0, c->stream,
                      c->partial + off, NORM_BLOCKS, c->scal + slot, c->mbox_dev + slot);
 }
@@ -401,8 +458,7 @@ static void launch_iteration(qdeq_ctx *c) {
   } else if (tiled) {
     int nparts = 0;
     vcycle_tiled_fp64(c, c->partial, &nparts);
-    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(NORM_THREADS), 0, c->stream,
-                       c->partial, nparts, c->scal + SC_NORM, c->mbox_dev + SC_NORM);
+    launch_norm_final(c, nparts);
   } else {
     vcycle<double, true>(c, c->d, 0, false);
     launch_resnorm(c, false);
@@ -423,7 +479,9 @@ static int ensure_graph(qdeq_ctx *c) {
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess) {
+    c->gate_ptr = c->gate;
     launch_iteration(c);
+    c->gate_ptr = nullptr;
     e = hipStreamEndCapture(c->stream, &g);
   }
   if (e == hipSuccess) e = hipGraphInstantiate(&c->g_cycle, g, nullptr, nullptr, 0);
@@ -542,90 +600,166 @@ static int mg_finish(qdeq_ctx *c, int st) {
   c->vcycles += m.it;
   if (m.ir) c->ir_sweeps += m.it;
   c->solves += 1;
+  // contraction per V-cycle seen by this solve (for the next batch sizes)
+  if (st == QDEQ_OK && m.it > 0 && m.delta0 > 0 && m.delta > 0) {
+    const double r = std::pow(m.delta / m.delta0, 1.0 / m.it);
+    if (r > 0 && r < 1) c->rho_est = 0.7 * c->rho_est + 0.3 * r;
+  }
   m.phase = 0;
   return st;
+}
+
+// Non-blocking (or blocking) check that the batch's last V-cycle has been
+// seen by the device: mb[MB_SEQ] reaches the batch's sequence number.
+static int seq_check(qdeq_ctx *c, long long target, bool block, int *ready) {
+  volatile double *p = reinterpret_cast<volatile double *>(c->mbox_host) + MB_SEQ;
+  for (;;) {
+    if ((long long)*p >= target) {
+      std::atomic_thread_fence(std::memory_order_acquire);
+      c->mb_idle = false;
+      *ready = 1;
+      return QDEQ_OK;
+    }
+    *ready = 0;
+    if ((++c->mb_polls & 255) == 0) {
+      const hipError_t e = hipStreamQuery(c->stream);
+      if (e != hipSuccess && e != hipErrorNotReady)
+        return set_error(c, QDEQ_ERR_HIP,
+                         std::string("waiting for V-cycles: ") + hipGetErrorString(e));
+      if (e == hipSuccess) {
+        const auto now = std::chrono::steady_clock::now();
+        if (!c->mb_idle) {
+          c->mb_idle = true;
+          c->mb_idle_since = now;
+        } else if (now - c->mb_idle_since > std::chrono::milliseconds(200)) {
+          c->mb_idle = false;
+          return set_error(c, QDEQ_ERR_STATE, "V-cycle sequence was not written");
+        }
+      }
+    }
+    if (!block) return QDEQ_OK;
+  }
+}
+
+// Launches up to k V-cycles back to back; each skips once the solve has
+// converged (or failed), so launching more than needed changes nothing.
+static int launch_batch(qdeq_ctx *c, int k) {
+  MgSolve &m = c->mg;
+  m.sample = false;
+  for (int i = 0; i < k; i++) {
+    // kernel timing samples every kTimingEvery-th V-cycle: that one is
+    // launched directly (same kernels and arguments as the graph) with the
+    // events recorded around the timed kernel
+    const bool sample = c->timing != 0 && c->timing != QDEQ_TIME_FAULT &&
+                        use_tiled(c) && (c->timed_seq++ % kTimingEvery) == 0;
+    if (sample) {
+      c->emit_events = true;
+      c->gate_ptr = c->gate;
+      launch_iteration(c);
+      c->gate_ptr = nullptr;
+      c->emit_events = false;
+      QDEQ_HIP(c, hipGetLastError());
+      m.sample = true;
+    } else {
+      QDEQ_HIP(c, hipGraphLaunch(c->g_cycle, c->stream));
+    }
+  }
+  c->seq_issued += k;
+  m.seq_target = c->seq_issued;
+  return QDEQ_OK;
+}
+
+// V-cycles to launch for the residual ratio left, from the observed
+// contraction (the tiled cycles only; one at a time otherwise or when
+// kernels are being timed).
+static int batch_size(const qdeq_ctx *c, double delta, double thr) {
+  if (!use_tiled(c) || (c->timing != 0 && c->timing != QDEQ_TIME_FAULT)) return 1;
+  const double rho = std::min(std::max(c->rho_est, 1e-3), 0.9);
+  if (!(delta > 0) || !(thr > 0)) return 1;
+  const double need = std::log(thr / delta) / std::log(rho);
+  int k = (int)std::ceil(need - 1e-9);
+  return std::min(std::max(k, 1), 6);
 }
 
 int mg_poll(qdeq_ctx *c, bool block, int *done) {
   MgSolve &m = c->mg;
   *done = 0;
-  int st, ready;
-  for (;;) {
-    if (m.phase == 1) {
-      if ((st = mbox_check(c, SC_NORM, block, &ready)) || !ready) break;
-      if (c->cfg.tol_mode == QDEQ_TOL_REL &&
-          ((st = mbox_check(c, SC_NORM0, block, &ready)) || !ready))
-        break;
-      if (m.cand && ((st = mbox_check(c, SC_NORM_PRED, block, &ready)) || !ready))
-        break;
-      for (int i : {SC_NORM, SC_NORM0, SC_NORM_PRED}) c->scal_host[i] = c->mbox_host[i];
-      m.delta = c->scal_host[SC_NORM];
-      if (m.cand) {
-        // used only if its residual is finite and below the previous solution's
-        const double pn = c->scal_host[SC_NORM_PRED];
-        if (std::isfinite(pn) && pn < m.delta) {
-          const LevelDims &d = c->lv[0];
-          QDEQ_HIP(c, hipMemcpyAsync(c->d[0].u, c->d[0].r,
-                                     (size_t)d.N * d.P * sizeof(double),
-                                     hipMemcpyDeviceToDevice, c->stream));
-          if (m.ir) launch_resnorm(c, true);  // FP32 residual of the new start
-          m.delta = pn;
-          c->g_accepted++;
-        }
+  int st = QDEQ_OK, ready;
+  if (m.phase == 1) {
+    if ((st = mbox_check(c, SC_NORM, block, &ready)) || !ready) goto out;
+    if (c->cfg.tol_mode == QDEQ_TOL_REL &&
+        ((st = mbox_check(c, SC_NORM0, block, &ready)) || !ready))
+      goto out;
+    if (m.cand && ((st = mbox_check(c, SC_NORM_PRED, block, &ready)) || !ready))
+      goto out;
+    for (int i : {SC_NORM, SC_NORM0, SC_NORM_PRED}) c->scal_host[i] = c->mbox_host[i];
+    m.delta = c->scal_host[SC_NORM];
+    if (m.cand) {
+      // used only if its residual is finite and below the previous solution's
+      const double pn = c->scal_host[SC_NORM_PRED];
+      if (std::isfinite(pn) && pn < m.delta) {
+        const LevelDims &d = c->lv[0];
+        QDEQ_HIP(c, hipMemcpyAsync(c->d[0].u, c->d[0].r,
+                                   (size_t)d.N * d.P * sizeof(double),
+                                   hipMemcpyDeviceToDevice, c->stream));
+        if (m.ir) launch_resnorm(c, true);  // FP32 residual of the new start
+        m.delta = pn;
+        c->g_accepted++;
       }
-      m.r0 = c->cfg.tol_mode == QDEQ_TOL_REL ? c->scal_host[SC_NORM0] : m.delta;
-      m.thr = c->cfg.tol_mode == QDEQ_TOL_REL ? c->cfg.tol * m.r0 : c->cfg.tol;
-    } else if (m.phase == 2) {
-      if ((st = mbox_check(c, SC_NORM, block, &ready)) || !ready) break;
-      if (m.sample) {
-        float ms = 0.0f;
-        QDEQ_HIP(c, hipEventSynchronize(c->ev_t1));
-        if (hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1) == hipSuccess) {
-          c->timed_ms += ms;
-          c->timed_calls += 1;
-        }
-      }
-      m.delta = c->scal_host[SC_NORM] = c->mbox_host[SC_NORM];
-      m.it++;
-      if (!std::isfinite(m.delta)) {
-        *done = 1;
-        return mg_finish(c, set_error(c, QDEQ_ERR_MG_NOT_CONVERGED,
-                                      "multigrid: residual is not finite"));
-      }
-    } else {
-      return set_error(c, QDEQ_ERR_STATE, "multigrid: no solve in progress");
     }
-    // decide: converged, capped, or one more V-cycle
+    m.delta0 = m.delta;
+    m.r0 = c->cfg.tol_mode == QDEQ_TOL_REL ? c->scal_host[SC_NORM0] : m.delta;
+    m.thr = c->cfg.tol_mode == QDEQ_TOL_REL ? c->cfg.tol * m.r0 : c->cfg.tol;
     if (!(m.delta > m.thr)) {
       *done = 1;
       return mg_finish(c, QDEQ_OK);
     }
-    if (m.it >= c->cfg.max_vcycles) {
+    if (c->cfg.max_vcycles < 1) {
+      *done = 1;
+      return mg_finish(c, set_error(c, QDEQ_ERR_MG_NOT_CONVERGED,
+                                    "multigrid: V-cycle cap reached"));
+    }
+    QDEQ_HIP(c, hipMemsetAsync(c->gate, 0, 3 * sizeof(int), c->stream));
+    if ((st = launch_batch(c, batch_size(c, m.delta, m.thr)))) goto out;
+    m.phase = 2;
+    if (!block) return QDEQ_OK;
+  }
+  for (;;) {
+    if (m.phase != 2) return set_error(c, QDEQ_ERR_STATE, "multigrid: no solve in progress");
+    if ((st = seq_check(c, m.seq_target, block, &ready)) || !ready) goto out;
+    if (m.sample) {
+      float ms = 0.0f;
+      QDEQ_HIP(c, hipEventSynchronize(c->ev_t1));
+      if (hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1) == hipSuccess) {
+        c->timed_ms += ms;
+        c->timed_calls += 1;
+      }
+      m.sample = false;
+    }
+    const int code = (int)c->mbox_host[MB_CODE];
+    m.it = (int)c->mbox_host[MB_COUNT];
+    m.delta = c->scal_host[SC_NORM] = c->mbox_host[SC_NORM];
+    if (code == 2) {
+      *done = 1;
+      return mg_finish(c, set_error(c, QDEQ_ERR_MG_NOT_CONVERGED,
+                                    "multigrid: residual is not finite"));
+    }
+    if (code == 3) {
       *done = 1;
       return mg_finish(c, set_error(c, QDEQ_ERR_MG_NOT_CONVERGED,
                                     "multigrid: V-cycle cap reached (residual " +
                                         std::to_string(m.delta) + " > " +
                                         std::to_string(m.thr) + ")"));
     }
-    mbox_arm(c, SC_NORM);
-    // kernel timing samples every kTimingEvery-th V-cycle: that one is
-    // launched directly (same kernels and arguments as the graph) with the
-    // events recorded around the timed kernel
-    m.sample = c->timing != 0 && use_tiled(c) && (c->timed_seq++ % kTimingEvery) == 0;
-    if (m.sample) {
-      c->emit_events = true;
-      launch_iteration(c);
-      c->emit_events = false;
-      QDEQ_HIP(c, hipGetLastError());
-    } else {
-      QDEQ_HIP(c, hipGraphLaunch(c->g_cycle, c->stream));
+    if (code == 1) {
+      *done = 1;
+      return mg_finish(c, QDEQ_OK);
     }
-    m.phase = 2;
-    if (!block) {
-      // the norm can only be ready once the V-cycle ran: return now
-      return QDEQ_OK;
-    }
+    // not there yet: another batch sized from the residual left
+    if ((st = launch_batch(c, batch_size(c, m.delta, m.thr)))) goto out;
+    if (!block) return QDEQ_OK;
   }
+out:
   if (st != QDEQ_OK) {
     *done = 1;
     return mg_finish(c, st);

@@ -163,7 +163,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_down(
     const T *__restrict__ uin, T *__restrict__ uout, const T *__restrict__ f,
     const T *__restrict__ cx, const T *__restrict__ cz, int N, int K, int P,
     T DH2, T relax, int nu, int zero_u, T *__restrict__ r2, int Nc, int Kc,
-    int Pc, int TX, int TY) {
+    int Pc, int TX, int TY, const int *__restrict__ gate) {
+  if (gate && gate[0]) return;  // the solve has converged: skip this cycle
   extern __shared__ __align__(16) unsigned char smem[];
   const int H = 2 * nu + 2;
   Tile<T> t = make_tile<T>(smem, TX, TY, H, HAS_F);
@@ -238,7 +239,9 @@ __global__ __launch_bounds__(TILE_THREADS) void k_up(
     const T *__restrict__ uin, T *__restrict__ uout, const T *__restrict__ f,
     const T *__restrict__ cx, const T *__restrict__ cz, int N, int K, int P,
     T DH2, T relax, int nu, const T *__restrict__ e, int Nc, int Kc, int Pc,
-    int TX, int TY, double *__restrict__ partial, double *__restrict__ u64) {
+    int TX, int TY, double *__restrict__ partial, double *__restrict__ u64,
+    const int *__restrict__ gate) {
+  if (gate && gate[0]) return;  // the solve has converged: skip this cycle
   extern __shared__ __align__(16) unsigned char smem[];
   const int H = 2 * nu + (MODE == UP_NORM ? 1 : 0);
   Tile<T> t = make_tile<T>(smem, TX, TY, H, HAS_F);
@@ -319,6 +322,7 @@ struct TailArgs {
   double relax;
   const void *minv;     // exact coarsest solve (column-major inverse) or null
   int cn;               // coarsest unknowns for minv
+  const int *gate;      // skip flag of a batched solve, or null
 };
 
 template <typename T>
@@ -362,6 +366,7 @@ __device__ inline void tail_smooth(const TailBufs<T> &b, int N, int K, T DH2,
 
 template <typename T, bool EXACT, bool FULL_TOP>
 __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailArgs a) {
+  if (a.gate && a.gate[0]) return;  // the solve has converged
   extern __shared__ __align__(16) unsigned char smem[];
   T *base = reinterpret_cast<T *>(smem);
   TailBufs<T> B[6];
@@ -580,12 +585,12 @@ static void launch_down(qdeq_ctx *c, LevelBufs<T> *B, int l, bool has_f,
     hipLaunchKernelGGL((k_down<T, true, EXACT>), grd, blk, lds,
                        c->stream, B[l].u, B[l].r, B[l].f, B[l].cx, B[l].cz, d.N,
                        d.K, d.P, DH2, relax, nu, zero_u ? 1 : 0, B[l + 1].f,
-                       dc.N, dc.K, dc.P, TX, TY);
+                       dc.N, dc.K, dc.P, TX, TY, (const int *)c->gate_ptr);
   else
     hipLaunchKernelGGL((k_down<T, false, EXACT>), grd, blk, lds,
                        c->stream, B[l].u, B[l].r, B[l].f, B[l].cx, B[l].cz, d.N,
                        d.K, d.P, DH2, relax, nu, zero_u ? 1 : 0, B[l + 1].f,
-                       dc.N, dc.K, dc.P, TX, TY);
+                       dc.N, dc.K, dc.P, TX, TY, (const int *)c->gate_ptr);
 }
 
 template <typename T, bool EXACT, bool HAS_F, int MODE>
@@ -604,7 +609,7 @@ static void launch_up_t(qdeq_ctx *c, LevelBufs<T> *B, int l, double *partial,
   hipLaunchKernelGGL((k_up<T, HAS_F, EXACT, true, MODE>), grd, dim3(TX_THREADS, sh.tyt),
                      lds, c->stream, B[l].r, B[l].u, B[l].f, B[l].cx, B[l].cz,
                      d.N, d.K, d.P, DH2, relax, nu, B[l + 1].u, dc.N, dc.K, dc.P,
-                     TX, TY, partial, u64);
+                     TX, TY, partial, u64, (const int *)c->gate_ptr);
 }
 
 template <typename T, bool EXACT>
@@ -625,6 +630,7 @@ static void launch_tail(qdeq_ctx *c, LevelBufs<T> *B, int lt) {
   a.minv = c->coarse_n > 0 ? (sizeof(T) == 8 ? (const void *)c->cinv64
                                              : (const void *)c->cinv32)
                            : nullptr;
+  a.gate = c->gate_ptr;
   hipLaunchKernelGGL((k_tail<T, EXACT, true>), dim3(1), dim3(TAIL_THREADS),
                      tail_lds_bytes<T>(c, lt), c->stream, a);
 }

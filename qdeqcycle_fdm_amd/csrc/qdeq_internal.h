@@ -59,17 +59,21 @@ enum {
   SC_VMAX = 3,
   SC_TSMAX = 4,
   SC_NORM_PRED = 5,  // residual norm of the learned-guess candidate
-  SC_COUNT = 8
+  MB_SEQ = 8,        // mailbox only: sequence number of the last V-cycle seen
+  MB_COUNT = 9,      //   V-cycles done in the current solve
+  MB_CODE = 10,      //   outcome of the last V-cycle (k_norm_gate)
+  SC_COUNT = 16
 };
 
 struct OutputRing;  // output.hip
 
 // State of a solve in progress (mg_begin / mg_poll).
 struct MgSolve {
-  int phase = 0;        // 0 idle, 1 initial norms pending, 2 V-cycle pending
+  int phase = 0;        // 0 idle, 1 initial norms pending, 2 V-cycles pending
   bool ir = false, cand = false, sample = false;
   int it = 0;
-  double delta = 0, r0 = 0, thr = 0;
+  double delta = 0, delta0 = 0, r0 = 0, thr = 0;
+  long long seq_target = 0;  // sequence number ending the launched batch
 };
 
 // State of the resumable RK43 driver (api.hip).
@@ -148,6 +152,13 @@ struct qdeq_ctx {
   // resumable solve and RK43 driver; mailbox polling state
   qdeq::MgSolve mg;
   qdeq::RkMachine rk;
+  // batched V-cycles: device gate (stop, count, outcome, sequence), the gate
+  // the kernels being launched check (set while capturing the V-cycle), the
+  // sequence numbers issued, and the observed V-cycle contraction
+  int *gate = nullptr;
+  int *gate_ptr = nullptr;
+  long long seq_issued = 0;
+  double rho_est = 0.1;
   long long mb_polls = 0;
   // output records and event catalogue (output.hip)
   qdeq::OutputRing *out = nullptr;

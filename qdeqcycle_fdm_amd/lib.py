@@ -52,7 +52,7 @@ EXPORTS = (
     "qdeq_guess_forward", "qdeq_attach_guess", "qdeq_guess_stats",
     "qdeq_guess_inject", "qdeq_guess_infer",
 )
-TIME_OFF, TIME_UP0, TIME_TAIL, TIME_DOWN1 = 0, 1, 2, 3
+TIME_OFF, TIME_UP0, TIME_TAIL, TIME_DOWN1, TIME_FAULT = 0, 1, 2, 3, 4
 GUESS_OFF, GUESS_HARVEST, GUESS_PREDICT = 0, 1, 2
 
 
