@@ -274,6 +274,78 @@ def run_ensemble(members, steps, precision_mode):
                 bitwise_equal_to_alone=bool(same))
 
 
+def run_mixed_2049(n_events=2, tail_steps=200):
+    """Mixed-precision workload: the basin model on 2049^2 (8 levels) with
+    FP32 V-cycles in FP64 refinement, against the FP64 cycle on the same
+    grid (the precision error) and the 1025^2 FP64 run (the resolution
+    difference).  Every run starts at t = 0.  The 1025^2 FP64 run steps until
+    n_events events (max V > 1e-3 m/s) have ended and then tail_steps more,
+    into the quiet phase; its time T is where the others stop exactly.
+    Reported: rates, slip and log10 V differences on the common fault nodes
+    at T, event onsets and recurrence intervals."""
+    import numpy as np
+    from qdeqcycle_fdm_amd import config, lib
+
+    def run(n, mode, tmax=None):
+        levels = {1025: 7, 2049: 8}[n]
+        pb = config.make_bp1_problem(n, basin=True, levels=levels,
+                                     tol_mode=config.TOL_REL, tol=1e-10)
+        me = lib.build_mu_equi(pb.mu)
+        extra = {} if tmax is None else {"tmax": tmax}
+        eng = make_engine(pb, me, mode, **extra)
+        eng.rk43_setup()
+        eng.events_enable(1e-3)
+        sync()
+        t0 = time.perf_counter()
+        if tmax is not None:
+            info = eng.rk43_step(10 ** 7)
+        else:
+            # until n_events events have ended, then tail_steps more
+            while True:
+                info = eng.rk43_step(100)
+                done = [e for e in eng.events() if e["t_end"] >= 0]
+                if len(done) >= n_events or info["it"] > 20000:
+                    break
+            info = eng.rk43_step(tail_steps)
+        sync()
+        sec = time.perf_counter() - t0
+        st = eng.state()
+        ev = eng.events()
+        eng.close()
+        log(f"{n}^2 {mode}: {info['it']} steps to t={info['t']:.6g} in {sec:.1f} s")
+        return dict(info=info, seconds=sec, slip=st["slip"][1:-1],
+                    V=st["V"][1:-1], events=ev)
+
+    ref = run(1025, "fast64")
+    T = ref["info"]["t"]
+    ir = run(2049, "ir32", tmax=T)
+    f64 = run(2049, "fast64", tmax=T)
+
+    def cmp(a, b, stride_a=1, stride_b=1):
+        sa, sb = a["slip"][::stride_a], b["slip"][::stride_b]
+        va = np.log10(np.maximum(a["V"][::stride_a], 1e-300))
+        vb = np.log10(np.maximum(b["V"][::stride_b], 1e-300))
+        ea = [e["t_onset"] for e in a["events"]]
+        eb = [e["t_onset"] for e in b["events"]]
+        k = min(len(ea), len(eb))
+        return dict(slip_max_rel=float(np.max(np.abs(sa - sb)) / np.max(np.abs(sb))),
+                    log10V_max_abs=float(np.max(np.abs(va - vb))),
+                    onsets=[ea, eb],
+                    onset_rel_diff=[abs(ea[i] - eb[i]) / eb[i] for i in range(k)],
+                    recurrence=[[ea[i + 1] - ea[i] for i in range(len(ea) - 1)],
+                                [eb[i + 1] - eb[i] for i in range(len(eb) - 1)]])
+
+    def rate(r):
+        return dict(steps=r["info"]["it"], seconds=round(r["seconds"], 2),
+                    steps_per_s=r["info"]["it"] / r["seconds"],
+                    vcycles_per_solve=r["info"]["vcycles"] / max(r["info"]["solves"], 1))
+
+    return dict(T=T, runs={"1025_fp64": rate(ref), "2049_ir32": rate(ir),
+                           "2049_fp64": rate(f64)},
+                ir32_vs_fp64_2049=cmp(ir, f64),
+                ir32_2049_vs_fp64_1025=cmp(ir, ref, stride_a=2))
+
+
 def cpu_baseline(pb, me, eng, seconds):
     """Oracle restarted from the engine's state; the engine then runs the
     same steps; returns the CPU rate, the GPU rate on the same steps and
@@ -325,6 +397,9 @@ def main():
                     help="only run the ensemble workload: M members, "
                          "--ensemble-steps steps each, alone and together")
     ap.add_argument("--ensemble-steps", type=int, default=1000)
+    ap.add_argument("--mixed2049", action="store_true",
+                    help="only run the 2049^2 mixed-precision workload and its "
+                         "accuracy comparison")
     ap.add_argument("--ensemble-mode", default="ir32", choices=("fast64", "ir32"))
     ap.add_argument("--no-autotune", action="store_true",
                     help="default tile shapes instead of the autotuned ones")
@@ -357,6 +432,15 @@ def main():
         os.environ["GPU_MAX_HW_QUEUES"] = "16"
     from qdeqcycle_fdm_amd import config, lib
     lib.load()
+    if args.mixed2049:
+        res = run_mixed_2049()
+        r = res["runs"]["2049_ir32"]
+        print(json.dumps({"metric": "RK43 steps/s at 2049^2, FP32 V-cycles in FP64 "
+                                    "refinement, 1e-10 relative residual",
+                          "value": round(r["steps_per_s"], 3), "unit": "steps/s",
+                          "n_gpus": 1, "higher_is_better": True, "dtype": "f32+f64",
+                          "data": "synthetic", "mixed2049": res}), flush=True)
+        return
     if args.ensemble:
         res = run_ensemble(args.ensemble, args.ensemble_steps, args.ensemble_mode)
         log(f"ensemble: {res['steps_per_s_ensemble']:.1f} steps/s together, "

@@ -216,3 +216,24 @@ def test_kernel_timing_does_not_change_results():
         assert info["t"] == out[0][0]["t"] and info["vcycles"] == out[0][0]["vcycles"]
         for k in ("slip", "psi", "U"):
             np.testing.assert_array_equal(st[k], out[0][1][k])
+
+
+def test_ir_and_fp64_at_2049():
+    """The mixed-precision workload's grid: 2049^2, 8 levels.  FP32 V-cycles
+    in FP64 refinement and the FP64 cycle (bit-checked against the oracle at
+    smaller sizes) both reach the 1e-10 relative residual on a perturbed
+    state, with autotuned tiles, and agree to what that residual implies."""
+    pb = config.make_bp1_problem(2049, basin=True, levels=8,
+                                 tol_mode=config.TOL_REL, tol=1e-10)
+    U = _perturbed(pb, 11, scale=0.05)
+    out = {}
+    for prec, tile in ((config.PREC_FP64, (64, 16)), (config.PREC_FP32_IR, (64, 32))):
+        with lib.Engine(pb, **{**FAST, "precision": prec, "max_vcycles": 60}) as eng:
+            eng.set_tiles(True, *tile)
+            eng.autotune_tiles(2)
+            V = U.copy()
+            vc, r0, r = eng.ode_solve(V)
+            assert r <= 1e-10 * r0 and 1 <= vc <= 60
+            out[prec] = V
+    scale = np.max(np.abs(out[config.PREC_FP64]))
+    assert np.max(np.abs(out[config.PREC_FP32_IR] - out[config.PREC_FP64])) <= 1e-6 * scale
