@@ -330,6 +330,27 @@ struct TailBufs {
   T *u, *f, *cx, *cz;  // compact, stride K; cz has N+1 rows
 };
 
+// LDS arrays of tail level l, recomputed from the level dimensions (uniform
+// values) at each use: a per-level table indexed by a loop variable would
+// live in per-lane scratch memory.
+template <typename T>
+__device__ inline TailBufs<T> tail_bufs(T *base, const TailArgs &a, int l) {
+  size_t off = 0;
+  for (int i = 0; i < l; i++)
+    off += 4 * (size_t)a.lv[i].N * a.lv[i].K + a.lv[i].K;
+  const size_t n = (size_t)a.lv[l].N * a.lv[l].K;
+  return TailBufs<T>{base + off, base + off + n, base + off + 2 * n,
+                     base + off + 3 * n};
+}
+
+template <typename T>
+__device__ inline T *tail_end(T *base, const TailArgs &a) {
+  size_t off = 0;
+  for (int i = 0; i < a.nlev; i++)
+    off += 4 * (size_t)a.lv[i].N * a.lv[i].K + a.lv[i].K;
+  return base + off;
+}
+
 // Sweeps on one tail level.  With one_wave, the first wavefront alone runs
 // them (small coarsest grids), ordering its own LDS traffic between
 // half-sweeps with wavefront fences instead of workgroup barriers.
@@ -369,38 +390,30 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailArgs a) {
   if (a.gate && a.gate[0]) return;  // the solve has converged
   extern __shared__ __align__(16) unsigned char smem[];
   T *base = reinterpret_cast<T *>(smem);
-  TailBufs<T> B[6];
-  size_t off = 0;
-  for (int l = 0; l < a.nlev; l++) {
-    const int n = a.lv[l].N * a.lv[l].K;
-    B[l].u = base + off;
-    B[l].f = base + off + n;
-    B[l].cx = base + off + 2 * n;
-    B[l].cz = base + off + 3 * n;
-    off += 4 * (size_t)n + a.lv[l].K;
-  }
-  T *Rs = base + off;  // residual scratch (entry level size)
+  T *Rs = tail_end(base, a);  // residual scratch (entry level size)
   T *Ps = Rs + (a.lv[0].N * a.lv[0].K > a.cn ? a.lv[0].N * a.lv[0].K : a.cn);
   // Ps: 1024 partial sums of the exact coarsest solve
   const int tid = threadIdx.x;
   // coefficients of every tail level, and the entry level's right-hand side
   for (int l = 0; l < a.nlev; l++) {
     const TailLevel &L = a.lv[l];
+    const TailBufs<T> b = tail_bufs(base, a, l);
     const T *cx = reinterpret_cast<const T *>(L.cx);
     const T *cz = reinterpret_cast<const T *>(L.cz);
     for (int idx = tid; idx < (L.N + 1) * L.K; idx += blockDim.x) {
       const int j = idx / L.K, k = idx - j * L.K;
-      B[l].cz[idx] = cz[(size_t)j * L.P + k];
-      if (j < L.N) B[l].cx[idx] = cx[(size_t)j * L.P + k];
+      b.cz[idx] = cz[(size_t)j * L.P + k];
+      if (j < L.N) b.cx[idx] = cx[(size_t)j * L.P + k];
     }
   }
   {
     const TailLevel &L0 = a.lv[0];
+    const TailBufs<T> b = tail_bufs(base, a, 0);
     const T *fin = reinterpret_cast<const T *>(a.f_in);
     for (int idx = tid; idx < L0.N * L0.K; idx += blockDim.x) {
       const int j = idx / L0.K, k = idx - j * L0.K;
-      B[0].u[idx] = T(0);
-      B[0].f[idx] = (k >= 1 && k <= L0.K - 2) ? fin[(size_t)j * L0.P + k] : T(0);
+      b.u[idx] = T(0);
+      b.f[idx] = (k >= 1 && k <= L0.K - 2) ? fin[(size_t)j * L0.P + k] : T(0);
     }
   }
   __syncthreads();
@@ -408,6 +421,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailArgs a) {
   for (int l = 0; l < a.nlev; l++) {
     const TailLevel &L = a.lv[l];
     const T DH2 = (T)L.DH2;
+    const TailBufs<T> Bl = tail_bufs(base, a, l);
     if (l == a.nlev - 1) {
       if (a.minv) {
         // e = B^{-1} (DH^2 f), rows summed in column order
@@ -415,28 +429,28 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailArgs a) {
         const int W = L.K - 2;
         for (int i = tid; i < a.cn; i += blockDim.x) {
           const int j = i / W, k = i - j * W + 1;
-          Rs[i] = DH2 * B[l].f[j * L.K + k];
+          Rs[i] = DH2 * Bl.f[j * L.K + k];
         }
         __syncthreads();
-        coarse_matvec<T>(minv, Rs, Ps, a.cn, B[l].u, W, L.K, (size_t)L.K);
+        coarse_matvec<T>(minv, Rs, Ps, a.cn, Bl.u, W, L.K, (size_t)L.K);
       } else {
         const bool one_wave = L.N * L.K <= 512;
-        tail_smooth<T, EXACT>(B[l], L.N, L.K, DH2, relax, a.nu_coarse, one_wave);
+        tail_smooth<T, EXACT>(Bl, L.N, L.K, DH2, relax, a.nu_coarse, one_wave);
       }
       __syncthreads();
       break;
     }
-    tail_smooth<T, EXACT>(B[l], L.N, L.K, DH2, relax, a.nu_pre, false);
+    tail_smooth<T, EXACT>(Bl, L.N, L.K, DH2, relax, a.nu_pre, false);
     for (int idx = tid; idx < L.N * L.K; idx += blockDim.x) {
       const int j = idx / L.K, k = idx - j * L.K;
       T r = T(0);
       if (k >= 1 && k <= L.K - 2) {
         const int jS = (j == 0) ? 1 : j - 1;
         const int jN = (j == L.N - 1) ? L.N - 2 : j + 1;
-        const T m1 = B[l].cx[idx], m2 = B[l].cx[idx + 1], m3 = B[l].cz[idx],
-                m4 = B[l].cz[idx + L.K];
-        const T *u = B[l].u;
-        r = B[l].f[idx] - (m1 * u[idx - 1] + m2 * u[idx + 1] + m3 * u[jS * L.K + k] +
+        const T m1 = Bl.cx[idx], m2 = Bl.cx[idx + 1], m3 = Bl.cz[idx],
+                m4 = Bl.cz[idx + L.K];
+        const T *u = Bl.u;
+        r = Bl.f[idx] - (m1 * u[idx - 1] + m2 * u[idx + 1] + m3 * u[jS * L.K + k] +
                            m4 * u[jN * L.K + k] - (m1 + m2 + m3 + m4) * u[idx]) /
                               DH2;
       }
@@ -444,6 +458,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailArgs a) {
     }
     __syncthreads();
     const TailLevel &C = a.lv[l + 1];
+    const TailBufs<T> Bc = tail_bufs(base, a, l + 1);
     for (int idx = tid; idx < C.N * C.K; idx += blockDim.x) {
       const int j = idx / C.K, k = idx - j * C.K;
       T v = T(0);
@@ -459,19 +474,20 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailArgs a) {
              T(2.0) * rs[kf] + T(2.0) * rn[kf] + rs[kf - 1] + rs[kf + 1] +
              rn[kf - 1] + rn[kf + 1]);
       }
-      B[l + 1].f[idx] = v;
-      B[l + 1].u[idx] = T(0);
+      Bc.f[idx] = v;
+      Bc.u[idx] = T(0);
     }
     __syncthreads();
   }
   for (int l = a.nlev - 2; l >= 0; l--) {
     const TailLevel &L = a.lv[l];
     const TailLevel &C = a.lv[l + 1];
+    const TailBufs<T> Bl = tail_bufs(base, a, l), Bc = tail_bufs(base, a, l + 1);
     const int jmax = FULL_TOP ? C.N : C.N - 1;
     for (int idx = tid; idx < jmax * (C.K - 1); idx += blockDim.x) {
       const int j = idx / (C.K - 1), k = idx - j * (C.K - 1);
-      const T *ec = B[l + 1].u + j * C.K + k;
-      T *uf = B[l].u + (2 * j) * L.K + 2 * k;
+      const T *ec = Bc.u + j * C.K + k;
+      T *uf = Bl.u + (2 * j) * L.K + 2 * k;
       const T e00 = ec[0], e01 = ec[1];
       uf[0] += e00;
       uf[1] += T(0.5) * (e00 + e01);
@@ -483,14 +499,15 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailArgs a) {
       }
     }
     __syncthreads();
-    tail_smooth<T, EXACT>(B[l], L.N, L.K, (T)L.DH2, relax, a.nu_post, false);
+    tail_smooth<T, EXACT>(Bl, L.N, L.K, (T)L.DH2, relax, a.nu_post, false);
   }
   {
     const TailLevel &L0 = a.lv[0];
     T *eo = reinterpret_cast<T *>(a.e_out);
+    const T *u0 = tail_bufs(base, a, 0).u;
     for (int idx = tid; idx < L0.N * L0.K; idx += blockDim.x) {
       const int j = idx / L0.K, k = idx - j * L0.K;
-      eo[(size_t)j * L0.P + k] = B[0].u[idx];
+      eo[(size_t)j * L0.P + k] = u0[idx];
     }
   }
 }
