@@ -248,6 +248,29 @@ int qdeq_events_enable(qdeq_ctx *ctx, double v_threshold, int capacity);
 /* Copies up to max entries; *n = events recorded (may exceed capacity). */
 int qdeq_events_get(qdeq_ctx *ctx, qdeq_event *out, int max, int *n);
 
+/* Projection start of the RK43 stage solves (no reference counterpart: the
+ * reference starts every solve from U as the previous solve left it,
+ * odeSolver.cpp:652-720, which stays the fallback).  The last `slots`
+ * solutions (1..8; 0 disables) are kept with their fault columns; each
+ * solve is offered the combination of them whose fault data best fit the
+ * new fault data in the least-squares sense, plus the exact response to the
+ * remote data, and uses it when its residual norm is finite and below the
+ * fallback's.  Enabling runs one solve (the unit remote response) and
+ * allocates slots + 1 level-0 fields; exclusive with qdeq_attach_guess. */
+int qdeq_projection_enable(qdeq_ctx *ctx, int slots);
+/* The least-squares fit of the projection start on host arrays, by the
+ * kernel the RK43 driver uses (for checking it): g_new (N = fault nodes of
+ * the context), g_old (cnt x N, newest first, 2 <= cnt <= 8).  coef (cnt + 2
+ * doubles): coef[0] = 0, coef[i] = c_i for i = 1..cnt-1 (0 for a column
+ * left out), coef[cnt] = ||misfit|| / ||g_new - g_old[0]||, coef[cnt + 1] =
+ * columns kept. */
+int qdeq_projection_fit(qdeq_ctx *ctx, const double *g_new,
+                        const double *g_old, int cnt, double *coef);
+/* Starts offered and used (shared with the learned-guess counters), and the
+ * last fit's ||residual|| / ||fault data change|| and column count. */
+int qdeq_projection_stats(qdeq_ctx *ctx, long long *offered,
+                          long long *used, double *fit_ratio, int *columns);
+
 /* Copies the current state to host arrays (any may be NULL): U (nx*nz, with
  * ghosts as ode_solve leaves them), slip, psi, V, dpsi, Ts (nz). */
 int qdeq_rk43_get_state(qdeq_ctx *ctx, qdeq_step_info *info, double *U,

@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD_DIR = os.path.join(ROOT, "_build")
 LIB_PATH = os.path.join(HERE, "libqdeq.so")
 SOURCES = ["mg.hip", "mg_tiled.hip", "coarse.hip", "fault.hip", "medium.hip", "api.hip",
-           "nn.hip", "guess.hip", "output.hip"]
+           "nn.hip", "guess.hip", "output.hip", "proj.hip"]
 ARCH = os.environ.get("QDEQ_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

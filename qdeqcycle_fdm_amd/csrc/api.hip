@@ -94,6 +94,7 @@ static void free_ctx(qdeq_ctx *c) {
   if (c->ev_t0) (void)hipEventDestroy(c->ev_t0);
   if (c->ev_t1) (void)hipEventDestroy(c->ev_t1);
   guess_free_ctx(c);
+  proj_free(c);
   (void)qdeq_output_close(c);
   events_free(c);
   if (c->rk.ev) (void)hipEventDestroy(c->rk.ev);
@@ -609,6 +610,7 @@ int qdeq_rk43_setup(qdeq_ctx *c, const double *a, const double *b,
   c->rk.phase = 0;  // idle
   c->mg.phase = 0;
   guess_reset(c);
+  proj_reset(c);
   return QDEQ_OK;
 }
 
@@ -702,6 +704,7 @@ static int rk_advance(qdeq_ctx *c, bool block, int *done) {
       case RK_STAGE_BEGIN:  // boundary data, then the elliptic solve
         launch_stage(c, r.s, c->t, c->dt);
         if ((st = guess_before_solve(c))) return st;
+        if ((st = proj_before_solve(c))) return st;
         if ((st = mg_begin(c))) return st;
         r.phase = RK_SOLVING;
         break;
@@ -710,6 +713,7 @@ static int rk_advance(qdeq_ctx *c, bool block, int *done) {
         if ((st = mg_poll(c, block, &sdone))) return st;
         if (!sdone) return QDEQ_OK;
         if ((st = guess_after_solve(c))) return st;
+        if ((st = proj_after_solve(c))) return st;
         if (c->timing == QDEQ_TIME_FAULT && (c->timed_seq++ % 8) == 0) {
           QDEQ_HIP(c, hipEventRecord(c->ev_t0, c->stream));
           launch_fault_update(c, r.s);

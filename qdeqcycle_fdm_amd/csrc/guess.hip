@@ -426,8 +426,9 @@ void guess_free_ctx(qdeq_ctx *c) {
   c->guess = nullptr;
 }
 
-// W_R: solution for unit remote data and zero fault data, by one solve.
-static int compute_wr(qdeq_ctx *c) {
+// W_R: solution for unit remote data and zero fault data, by one solve to
+// a relative residual of 1e-12.
+int solve_unit_remote(qdeq_ctx *c, double *dst) {
   const LevelDims &d = c->lv[0];
   const size_t n = (size_t)d.N * d.P;
   if ((size_t)2 * c->nx * c->nz < n)
@@ -449,7 +450,7 @@ static int compute_wr(qdeq_ctx *c) {
   c->solves = ss;
   c->ir_sweeps = si;
   if (st != QDEQ_OK) return st;
-  QDEQ_HIP(c, hipMemcpyAsync(c->wr, c->d[0].u, n * sizeof(double),
+  QDEQ_HIP(c, hipMemcpyAsync(dst, c->d[0].u, n * sizeof(double),
                              hipMemcpyDeviceToDevice, c->stream));
   QDEQ_HIP(c, hipMemcpyAsync(c->d[0].u, c->stage_U, n * sizeof(double),
                              hipMemcpyDeviceToDevice, c->stream));
@@ -787,6 +788,9 @@ int qdeq_attach_guess(qdeq_ctx *c, qdeq_guess *g, int mode) {
   }
   if (mode != QDEQ_GUESS_HARVEST && mode != QDEQ_GUESS_PREDICT)
     return set_error(c, QDEQ_ERR_ARG, "bad guess mode");
+  if (c->proj_slots > 0)
+    return set_error(c, QDEQ_ERR_STATE,
+                     "learned guess and projection start are exclusive");
   if (g->cfg.device != c->cfg.device || g->nk != c->cfg.nk ||
       c->cfg.ni != c->cfg.nk || c->lv[0].N != g->nk || c->lv[0].K != g->nk)
     return set_error(c, QDEQ_ERR_ARG,
@@ -815,7 +819,7 @@ int qdeq_attach_guess(qdeq_ctx *c, qdeq_guess *g, int mode) {
       QDEQ_HIP(c, dalloc(&c->gh[l], (size_t)16 * (l == 0 ? g->F0 : g->H)));
     const double one = 1.0;
     QDEQ_HIP(c, hipMemcpy(c->gs + 2, &one, sizeof(double), hipMemcpyHostToDevice));
-    int st = compute_wr(c);
+    int st = solve_unit_remote(c, c->wr);
     if (st != QDEQ_OK) {
       guess_free_ctx(c);
       return st;

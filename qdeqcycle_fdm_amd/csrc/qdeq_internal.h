@@ -172,6 +172,12 @@ struct qdeq_ctx {
   std::chrono::steady_clock::time_point mb_idle_since{};
   long long g_predicted = 0, g_accepted = 0, g_harvested = 0, g_seen = 0;
   hipGraphExec_t g_pred = nullptr;
+  // projection start (proj.hip): ring of proj_slots stored solutions with
+  // the remote part removed (N x P each) and their fault columns, the fit's
+  // coefficients, and W_R
+  int proj_slots = 0, proj_count = 0, proj_head = 0;
+  double *proj_z = nullptr, *proj_g = nullptr, *proj_coef = nullptr,
+         *proj_wr = nullptr;
 };
 
 namespace qdeq {
@@ -202,6 +208,14 @@ int guess_before_solve(qdeq_ctx *c);
 int guess_after_solve(qdeq_ctx *c);
 void guess_reset(qdeq_ctx *c);
 void guess_free_ctx(qdeq_ctx *c);
+// W_R into dst: the solution for unit remote data and zero fault data
+int solve_unit_remote(qdeq_ctx *c, double *dst);
+
+// projection start (proj.hip)
+int proj_before_solve(qdeq_ctx *c);
+int proj_after_solve(qdeq_ctx *c);
+void proj_reset(qdeq_ctx *c);
+void proj_free(qdeq_ctx *c);
 
 // tiled fast cycle (mg_tiled.hip)
 int tail_entry_level(const qdeq_ctx *c);

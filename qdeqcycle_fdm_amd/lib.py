@@ -42,7 +42,8 @@ EXPORTS = (
     "qdeq_device_synchronize", "qdeq_autotune_tiles", "qdeq_get_tile_shapes",
     "qdeq_kernel_breakdown", "qdeq_set_tile_shapes", "qdeq_ensemble_step",
     "qdeq_output_open", "qdeq_output_close", "qdeq_events_enable",
-    "qdeq_events_get",
+    "qdeq_events_get", "qdeq_projection_enable", "qdeq_projection_stats",
+    "qdeq_projection_fit",
     # include/qdeq_guess.h
     "qdeq_guess_default_config", "qdeq_train_default_config",
     "qdeq_guess_create", "qdeq_guess_destroy", "qdeq_guess_last_error",
@@ -205,6 +206,9 @@ def load(path: str = LIB_PATH):
     L.qdeq_guess_stats.argtypes = [_vp, _llp, _llp, _llp]
     L.qdeq_guess_inject.argtypes = [_vp, ctypes.c_double]
     L.qdeq_guess_infer.argtypes = [_vp, _fp, _fp]
+    L.qdeq_projection_enable.argtypes = [_vp, ctypes.c_int]
+    L.qdeq_projection_stats.argtypes = [_vp, _llp, _llp, _dp, _ip]
+    L.qdeq_projection_fit.argtypes = [_vp, _dp, _dp, ctypes.c_int, _dp]
     for name in EXPORTS:
         getattr(L, name)  # every declared symbol must resolve
     _lib = L
@@ -599,6 +603,32 @@ def _engine_guess_stats(self) -> dict:
     return dict(predicted=p.value, accepted=a.value, harvested=h.value)
 
 
+def _engine_projection_enable(self, slots: int):
+    """Projection start of the RK43 stage solves from the last `slots`
+    solutions (0 disables); runs one solve for the unit remote response."""
+    self._check(load().qdeq_projection_enable(self.h, slots))
+
+
+def _engine_projection_stats(self) -> dict:
+    p, a = ctypes.c_longlong(0), ctypes.c_longlong(0)
+    r, n = ctypes.c_double(0.0), ctypes.c_int(0)
+    self._check(load().qdeq_projection_stats(self.h, ctypes.byref(p), ctypes.byref(a),
+                                             ctypes.byref(r), ctypes.byref(n)))
+    return dict(offered=p.value, used=a.value, fit_ratio=r.value, columns=n.value)
+
+
+def _engine_projection_fit(self, g_new, g_old):
+    """Coefficients of the projection start's least-squares fit (the RK43
+    driver's kernel) for g_new (N) against g_old (cnt x N, newest first):
+    returns (c (cnt,), misfit ratio, columns kept)."""
+    g_new = np.ascontiguousarray(g_new, dtype=np.float64)
+    g_old = np.ascontiguousarray(g_old, dtype=np.float64)
+    cnt = g_old.shape[0]
+    out = np.zeros(cnt + 2)
+    self._check(load().qdeq_projection_fit(self.h, _p(g_new), _p(g_old), cnt, _p(out)))
+    return out[:cnt], float(out[cnt]), int(out[cnt + 1])
+
+
 def _engine_guess_inject(self, factor: float):
     self._check(load().qdeq_guess_inject(self.h, factor))
 
@@ -613,4 +643,7 @@ def _engine_guess_infer(self, x) -> np.ndarray:
 Engine.attach_guess = _engine_attach_guess
 Engine.guess_stats = _engine_guess_stats
 Engine.guess_inject = _engine_guess_inject
+Engine.projection_enable = _engine_projection_enable
+Engine.projection_stats = _engine_projection_stats
+Engine.projection_fit = _engine_projection_fit
 Engine.guess_infer = _engine_guess_infer
