@@ -6,6 +6,7 @@
 // in the kernels of fault.hip and mg.hip.  The reference's exit() paths
 // become status codes.
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -235,6 +236,11 @@ int qdeq_create(const qdeq_config *cfg, const double *mu,
     free_ctx(c);
     return QDEQ_ERR_HIP;
   }
+  // QDEQ_DIRECT_LAUNCH=1: V-cycles are launched kernel by kernel instead of
+  // as the captured graph (same kernels and arguments), for profilers that
+  // cannot trace graph launches
+  const char *dl = std::getenv("QDEQ_DIRECT_LAUNCH");
+  c->direct_launch = dl && dl[0] == '1';
   *out = c;
   return QDEQ_OK;
 }

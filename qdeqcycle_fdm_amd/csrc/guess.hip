@@ -360,6 +360,14 @@ int guess_before_solve(qdeq_ctx *c) {
     c->harvest_pending = true;
     return QDEQ_OK;
   }
+  if (c->direct_launch) {
+    mbox_arm(c, SC_NORM_PRED);
+    launch_predict_chain(c);
+    QDEQ_HIP(c, hipGetLastError());
+    c->cand_pending = true;
+    c->g_predicted++;
+    return QDEQ_OK;
+  }
   if (!c->g_pred) {
     hipGraph_t gr = nullptr;
     hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);

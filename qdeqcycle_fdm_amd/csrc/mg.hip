@@ -660,6 +660,11 @@ static int launch_batch(qdeq_ctx *c, int k) {
       c->emit_events = false;
       QDEQ_HIP(c, hipGetLastError());
       m.sample = true;
+    } else if (c->direct_launch) {
+      c->gate_ptr = c->gate;
+      launch_iteration(c);
+      c->gate_ptr = nullptr;
+      QDEQ_HIP(c, hipGetLastError());
     } else {
       QDEQ_HIP(c, hipGraphLaunch(c->g_cycle, c->stream));
     }

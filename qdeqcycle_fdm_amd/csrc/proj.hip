@@ -13,8 +13,8 @@
 //     Z_0 + sum_i c_i (Z_i - Z_0) + r* W_R,
 // Z_0 the newest stored Z, where c minimises
 //     || (g* - g_0) - sum_i c_i (g_i - g_0) ||_2
-// over the stored fault columns (classical Gram-Schmidt applied twice, in
-// one workgroup, every sum in a fixed order).  The start's residual is then
+// over the stored fault columns (CholeskyQR applied twice, in one
+// workgroup, every sum in a fixed order).  The start's residual is then
 // the response to the part of g* - g_0 outside the stored span, plus the
 // stored solutions' own residuals.
 
@@ -28,10 +28,6 @@ namespace qdeq {
 
 constexpr int PROJ_THREADS = 256;
 constexpr int PROJ_MAX_SLOTS = 8;
-// a difference column whose norm after orthogonalisation falls below this
-// fraction of its own norm is left out of the fit
-constexpr double kProjDrop = 1e-10;
-
 struct ProjSlots {
   int s[PROJ_MAX_SLOTS];  // ring slots, newest first
   int cnt;
@@ -39,139 +35,189 @@ struct ProjSlots {
 
 namespace {
 
-// Sums v[0..nv) over the 256 threads, the same result in every thread:
-// within a wavefront a butterfly (lane l adds lane l^m for m = 32..1; the
-// partners compute a + b and b + a, which are equal), then the four
-// wavefront sums in wavefront order.  Loops run to the static bound so v
-// stays in registers.
-constexpr int PROJ_NV = PROJ_MAX_SLOTS + 1;
+// The fit is CholeskyQR applied twice to the augmented matrix
+// A = [D_1 .. D_{m-1} | d] (columns in LDS, N entries each): G1 = A^T A by
+// one pass over the entries and one tree sum, its Cholesky factor R1
+// (G1 = R1^T R1), Q = A R1^{-1} in place, G2 = Q^T Q, R2 = chol(G2) and
+// R = R2 R1; c solves R_DD c = R_Dd and ||d - D c|| = R_dd.  The small
+// factorisations run in registers in every thread alike.  A D column whose
+// part independent of the columns before it has a squared norm of at most
+// kProjDrop2 of its own in the first factorisation (a norm ratio of 1e-6)
+// is left out; the second factorisation restores orthogonality lost to the
+// conditioning of the first.
+//
+// Order of every sum (restated on the CPU for bit-for-bit checks):
+// * G_ab: thread t adds A_a[j] * A_b[j] for j = t, t+256, ... from 0.0; the
+//   256 partial sums are then added pairwise in LDS, entry t += entry t+s
+//   for s = 128, 64, ..., 1;
+// * Cholesky, column by column: R_ki = (G_ki - sum_{l<k} R_lk R_li) * w_k
+//   and p_i = G_ii - sum_{l<i} R_li R_li, terms subtracted in increasing l,
+//   with R_ii = sqrt(p_i) and w_i = 1 / R_ii; a column left out has a zero
+//   column of R and w = 0 (so its row of R is zero and every term that
+//   involves it is zero);
+// * Q_i = (A_i - sum_{k<i} R_ki Q_k) * w_i, terms subtracted in increasing k;
+// * R_ij = sum_{k=i..j} R2_ik R1_kj from 0.0 in increasing k;
+// * back substitution from the last D column to the first, each row
+//   subtracting R_ik c_k in increasing k; c = 0 for a column left out.
+constexpr double kProjDrop2 = 1e-12;
 
-// v[0..nq) and, with last, v[PROJ_MAX_SLOTS] are summed.
-__device__ inline void block_sums(double (&v)[PROJ_NV], int nq, bool last,
-                                  double *sh) {
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-#pragma unroll
-  for (int q = 0; q < PROJ_NV; q++) {
-    if (q < nq || (last && q == PROJ_MAX_SLOTS)) {
-      double x = v[q];
-#pragma unroll
-      for (int m = 32; m >= 1; m >>= 1) x = x + __shfl_xor(x, m, 64);
-      v[q] = x;
-      if (lane == 0) sh[w * PROJ_NV + q] = x;
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < PROJ_NV; q++)
-    if (q < nq || (last && q == PROJ_MAX_SLOTS))
-      v[q] = ((sh[q] + sh[PROJ_NV + q]) + sh[2 * PROJ_NV + q]) + sh[3 * PROJ_NV + q];
-  __syncthreads();
+// Packed upper-triangle index of (a, b), a <= b < M.
+template <int M>
+__host__ __device__ constexpr int pidx(int a, int b) {
+  return a * M - a * (a - 1) / 2 + (b - a);
 }
 
-// Orthogonalises x against Q columns 1..nq of lds twice (classical
-// Gram-Schmidt, then again on the remainder); with want_norm the first pass
-// also sums x.x.  r receives the two passes' coefficients added.
-__device__ inline void cgs2(double *lds, int N, int nq, double *x, double *sh,
-                            double (&r)[PROJ_NV], bool want_norm, double *nrm0) {
+// Gram entries of the M columns of A into Gs (LDS), then every thread reads
+// them into G.  Pt: M(M+1)/2 * 256 doubles of LDS.
+template <int M>
+__device__ inline void gram(const double *A, int N, double *Pt, double *Gs,
+                            double (&G)[M * (M + 1) / 2]) {
+  constexpr int NP = M * (M + 1) / 2;
   const int tid = threadIdx.x;
-  for (int pass = 0; pass < 2; pass++) {
-    double v[PROJ_NV];
+  {
+    double v[NP];
 #pragma unroll
-    for (int q = 0; q < PROJ_NV; q++) v[q] = 0.0;
-    const bool nn = want_norm && pass == 0;
+    for (int q = 0; q < NP; q++) v[q] = 0.0;
     for (int j = tid; j < N; j += PROJ_THREADS) {
-      const double xj = x[j];
+      double x[M];
 #pragma unroll
-      for (int q = 0; q < PROJ_MAX_SLOTS; q++)
-        if (q < nq) v[q] = v[q] + lds[(size_t)(q + 1) * N + j] * xj;
-      if (nn) v[PROJ_MAX_SLOTS] = v[PROJ_MAX_SLOTS] + xj * xj;
-    }
-    block_sums(v, nq, nn, sh);
-    if (nn) *nrm0 = sqrt(v[PROJ_MAX_SLOTS]);
-    for (int j = tid; j < N; j += PROJ_THREADS) {
-      double xj = x[j];
+      for (int a = 0; a < M; a++) x[a] = A[(size_t)a * N + j];
 #pragma unroll
-      for (int q = 0; q < PROJ_MAX_SLOTS; q++)
-        if (q < nq) xj = xj - v[q] * lds[(size_t)(q + 1) * N + j];
-      x[j] = xj;
+      for (int a = 0; a < M; a++)
+#pragma unroll
+        for (int b = a; b < M; b++) v[pidx<M>(a, b)] = v[pidx<M>(a, b)] + x[a] * x[b];
     }
 #pragma unroll
-    for (int q = 0; q < PROJ_MAX_SLOTS; q++)
-      if (q < nq) r[q] = pass == 0 ? v[q] : r[q] + v[q];
+    for (int q = 0; q < NP; q++) Pt[q * PROJ_THREADS + tid] = v[q];
+  }
+  __syncthreads();
+  for (int ls = 7; ls >= 0; ls--) {
+    const int s = 1 << ls;
+    for (int idx = tid; idx < NP * s; idx += PROJ_THREADS) {
+      const int q = idx >> ls, t = idx & (s - 1);
+      double *p = Pt + q * PROJ_THREADS;
+      p[t] = p[t] + p[t + s];
+    }
+    __syncthreads();
+  }
+  if (tid < NP) Gs[tid] = Pt[tid * PROJ_THREADS];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NP; q++) G[q] = Gs[q];
+  __syncthreads();
+}
+
+// Upper Cholesky factor (packed) of the M x M matrix G over the D columns
+// in keep (bit i) and column M-1 (d, always factored, R_dd =
+// sqrt(max(p, 0))).  With drop, D columns failing the rule are removed from
+// keep and their column of R is zero.
+template <int M>
+__device__ inline void cholesky(const double (&G)[M * (M + 1) / 2], bool drop,
+                                unsigned &keep, double (&R)[M * (M + 1) / 2],
+                                double (&inv)[M]) {
+#pragma unroll
+  for (int q = 0; q < M * (M + 1) / 2; q++) R[q] = 0.0;
+#pragma unroll
+  for (int i = 0; i < M; i++) {
+    inv[i] = 0.0;
+    // a column left out has w = 0, so its row of R is zero and its terms
+    // in the sums below are zero
+#pragma unroll
+    for (int k = 0; k < i; k++) {
+      double s = G[pidx<M>(k, i)];
+#pragma unroll
+      for (int l = 0; l < k; l++) s = s - R[pidx<M>(l, k)] * R[pidx<M>(l, i)];
+      R[pidx<M>(k, i)] = s * inv[k];
+    }
+    double p = G[pidx<M>(i, i)];
+#pragma unroll
+    for (int l = 0; l < i; l++) p = p - R[pidx<M>(l, i)] * R[pidx<M>(l, i)];
+    const bool used = i == M - 1 || ((keep >> i) & 1u);
+    const bool dropped =
+        !used || (i < M - 1 && drop &&
+                  !(G[pidx<M>(i, i)] > 0.0 && p > kProjDrop2 * G[pidx<M>(i, i)]));
+    if (dropped) {
+      keep &= ~(1u << i);
+#pragma unroll
+      for (int k = 0; k <= i; k++) R[pidx<M>(k, i)] = 0.0;
+    } else {
+      R[pidx<M>(i, i)] = p > 0.0 ? sqrt(p) : 0.0;
+      inv[i] = p > 0.0 ? 1.0 / R[pidx<M>(i, i)] : 0.0;
+    }
   }
 }
 
-__device__ inline double norm2(const double *x, int N, double *sh) {
-  double v[PROJ_NV];
-#pragma unroll
-  for (int q = 0; q < PROJ_NV; q++) v[q] = 0.0;
-  for (int j = threadIdx.x; j < N; j += PROJ_THREADS) v[0] = v[0] + x[j] * x[j];
-  block_sums(v, 1, false, sh);
-  return sqrt(v[0]);
-}
-
-// Coefficients of the start.  LDS: cnt columns of N doubles; column 0 holds
-// d = g* - g_0, column i >= 1 holds g_i - g_0 on entry, and the kept
-// orthonormal columns are packed into columns 1..nq as they are found (the
-// a-th kept column is written over column a + 1 <= i).  Each thread owns
-// the entries j = tid + 256 e of every column, so only the sums need the
-// workgroup.  Output coef[i] = c_i for the stored solution i = 1..cnt-1 (0
-// for a column left out), coef[PROJ_MAX_SLOTS] = ||d - fit|| / ||d||,
-// coef[PROJ_MAX_SLOTS + 1] = columns kept.
+// Coefficients of the start for M = cnt stored solutions.  LDS: M columns
+// of N doubles (column i < M-1 holds g_{i+1} - g_0, column M-1 holds
+// d = g* - g_0), then the tree sums.  Output coef[i] = c_i, the weight of
+// (Z_i - Z_0), i = 1..cnt-1 (0 for a column left out, and beyond cnt),
+// coef[PROJ_MAX_SLOTS] = ||d - fit|| / ||d||, coef[PROJ_MAX_SLOTS + 1] =
+// columns kept.
+template <int M>
 __global__ __launch_bounds__(PROJ_THREADS) void k_proj_coef(
     const double *__restrict__ u, int N, int P, const double *__restrict__ g,
     ProjSlots sl, double *__restrict__ coef) {
+  constexpr int NP = M * (M + 1) / 2;
   extern __shared__ __align__(16) double lds[];
-  __shared__ double sh[4 * PROJ_NV];
-  __shared__ double sR[PROJ_MAX_SLOTS * PROJ_MAX_SLOTS];
-  __shared__ double sb[PROJ_MAX_SLOTS], sc[PROJ_MAX_SLOTS];
-  __shared__ int skept[PROJ_MAX_SLOTS];
+  __shared__ double Gs[NP];
   const int tid = threadIdx.x;
-  const int cnt = sl.cnt;
+  double *A = lds;
+  double *Pt = lds + (size_t)M * N;
   const double *g0 = g + (size_t)sl.s[0] * N;
   for (int j = tid; j < N; j += PROJ_THREADS) {
     const double b0 = g0[j];
-    lds[j] = u[(size_t)j * P] - b0;
-    for (int i = 1; i < cnt; i++) lds[(size_t)i * N + j] = g[(size_t)sl.s[i] * N + j] - b0;
-  }
-  int nq = 0;
-  double r[PROJ_NV];
-  for (int i = 1; i < cnt; i++) {
-    double *x = lds + (size_t)i * N;
-    double nrm0 = 0.0;
-    cgs2(lds, N, nq, x, sh, r, true, &nrm0);
-    const double nrm = norm2(x, N, sh);
-    if (nrm > 0.0 && nrm > kProjDrop * nrm0) {
-      double *q = lds + (size_t)(nq + 1) * N;
-      for (int j = tid; j < N; j += PROJ_THREADS) q[j] = x[j] / nrm;
-      if (tid == 0) {
 #pragma unroll
-        for (int a = 0; a < PROJ_MAX_SLOTS; a++)
-          if (a < nq) sR[a * PROJ_MAX_SLOTS + nq] = r[a];
-        sR[nq * PROJ_MAX_SLOTS + nq] = nrm;
-        skept[nq] = i;
-      }
-      nq++;
+    for (int i = 0; i < M - 1; i++) A[(size_t)i * N + j] = g[(size_t)sl.s[i + 1] * N + j] - b0;
+    A[(size_t)(M - 1) * N + j] = u[(size_t)j * P] - b0;
+  }
+  __syncthreads();
+  unsigned keep = (1u << (M - 1)) - 1u;  // the D columns
+  double G[NP], R1[NP], inv[M];
+  gram<M>(A, N, Pt, Gs, G);
+  const double dd = G[pidx<M>(M - 1, M - 1)];
+  cholesky<M>(G, true, keep, R1, inv);
+  // Q = A R1^{-1} on each thread's own entries
+  for (int j = tid; j < N; j += PROJ_THREADS) {
+    double x[M];
+#pragma unroll
+    for (int i = 0; i < M; i++) {
+      double q = A[(size_t)i * N + j];
+#pragma unroll
+      for (int k = 0; k < i; k++) q = q - R1[pidx<M>(k, i)] * x[k];
+      x[i] = q * inv[i];
+      A[(size_t)i * N + j] = x[i];
     }
   }
-  // right-hand side b = Q^T d (twice, d updated in between)
-  double nd0 = 0.0;
-  cgs2(lds, N, nq, lds, sh, r, true, &nd0);
-  const double nd = norm2(lds, N, sh);
+  __syncthreads();
+  double R2[NP];
+  gram<M>(A, N, Pt, Gs, G);
+  cholesky<M>(G, false, keep, R2, inv);
+  // R = R2 R1 (into G), then the back substitution
+#pragma unroll
+  for (int i = 0; i < M; i++)
+#pragma unroll
+    for (int j2 = i; j2 < M; j2++) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = i; k <= j2; k++) s = s + R2[pidx<M>(i, k)] * R1[pidx<M>(k, j2)];
+      G[pidx<M>(i, j2)] = s;
+    }
+  double c[M];
+#pragma unroll
+  for (int i = M - 2; i >= 0; i--) {
+    double acc = G[pidx<M>(i, M - 1)];
+#pragma unroll
+    for (int k = i + 1; k < M - 1; k++) acc = acc - G[pidx<M>(i, k)] * c[k];
+    c[i] = ((keep >> i) & 1u) ? acc / G[pidx<M>(i, i)] : 0.0;
+  }
+  const int kept = __builtin_popcount(keep);
   if (tid == 0) {
+    coef[0] = 0.0;
 #pragma unroll
-    for (int a = 0; a < PROJ_MAX_SLOTS; a++)
-      if (a < nq) sb[a] = r[a];
-    // back substitution R c = b over the kept columns
-    for (int a = nq - 1; a >= 0; a--) {
-      double acc = sb[a];
-      for (int b = a + 1; b < nq; b++) acc = acc - sR[a * PROJ_MAX_SLOTS + b] * sc[b];
-      sc[a] = acc / sR[a * PROJ_MAX_SLOTS + a];
-    }
-    for (int i = 0; i < PROJ_MAX_SLOTS; i++) coef[i] = 0.0;
-    for (int a = 0; a < nq; a++) coef[skept[a]] = sc[a];
-    coef[PROJ_MAX_SLOTS] = nd0 > 0.0 ? nd / nd0 : 0.0;
-    coef[PROJ_MAX_SLOTS + 1] = (double)nq;
+    for (int i = 1; i < PROJ_MAX_SLOTS; i++) coef[i] = i < M ? c[i - 1] : 0.0;
+    const double rd = G[pidx<M>(M - 1, M - 1)];
+    coef[PROJ_MAX_SLOTS] = dd > 0.0 ? rd / sqrt(dd) : 0.0;
+    coef[PROJ_MAX_SLOTS + 1] = (double)kept;
   }
 }
 
@@ -211,7 +257,37 @@ __global__ void k_proj_commit(const double *__restrict__ u,
 
 }  // namespace
 
-static size_t proj_lds_bytes(int slots, int N) { return (size_t)slots * N * sizeof(double); }
+static size_t proj_lds_bytes(int slots, int N) {
+  return ((size_t)slots * N + (size_t)slots * (slots + 1) / 2 * PROJ_THREADS) *
+         sizeof(double);
+}
+
+using CoefKernel = void (*)(const double *, int, int, const double *, ProjSlots, double *);
+
+static CoefKernel coef_kernel(int cnt) {
+  switch (cnt) {
+    case 2: return k_proj_coef<2>;
+    case 3: return k_proj_coef<3>;
+    case 4: return k_proj_coef<4>;
+    case 5: return k_proj_coef<5>;
+    case 6: return k_proj_coef<6>;
+    case 7: return k_proj_coef<7>;
+    default: return k_proj_coef<8>;
+  }
+}
+
+// Opts every fit kernel up to cnt columns into the LDS it needs.
+static hipError_t coef_lds_attr(int cnt, int N) {
+  for (int k = 2; k <= cnt; k++) {
+    const size_t b = proj_lds_bytes(k, N);
+    if (b <= 65536) continue;
+    const hipError_t e = hipFuncSetAttribute((const void *)coef_kernel(k),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)b);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
 
 void proj_reset(qdeq_ctx *c) {
   c->proj_count = 0;
@@ -240,7 +316,7 @@ int proj_before_solve(qdeq_ctx *c) {
   const ProjSlots sl = proj_order(c);
   mbox_arm(c, SC_NORM_PRED);
   if (sl.cnt >= 2)
-    hipLaunchKernelGGL(k_proj_coef, dim3(1), dim3(PROJ_THREADS),
+    hipLaunchKernelGGL(coef_kernel(sl.cnt), dim3(1), dim3(PROJ_THREADS),
                        proj_lds_bytes(sl.cnt, d.N), c->stream, c->d[0].u, d.N, d.P,
                        c->proj_g, sl, c->proj_coef);
   const dim3 blk(256), grd((d.P + 255) / 256, d.N);
@@ -296,10 +372,7 @@ int qdeq_projection_enable(qdeq_ctx *c, int slots) {
   QDEQ_HIP(c, hipMalloc(&c->proj_coef, (PROJ_MAX_SLOTS + 2) * sizeof(double)));
   QDEQ_HIP(c, hipMalloc(&c->proj_wr, n0 * sizeof(double)));
   QDEQ_HIP(c, hipMemset(c->proj_coef, 0, (PROJ_MAX_SLOTS + 2) * sizeof(double)));
-  if (proj_lds_bytes(slots, d.N) > 65536)
-    QDEQ_HIP(c, hipFuncSetAttribute((const void *)k_proj_coef,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)proj_lds_bytes(slots, d.N)));
+  QDEQ_HIP(c, coef_lds_attr(slots, d.N));
   int st = solve_unit_remote(c, c->proj_wr);
   if (st != QDEQ_OK) {
     proj_free(c);
@@ -323,10 +396,7 @@ int qdeq_projection_fit(qdeq_ctx *c, const double *g_new, const double *g_old,
   hipError_t e = hipMalloc(&u, (size_t)N * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&g, (size_t)cnt * N * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&cf, (PROJ_MAX_SLOTS + 2) * sizeof(double));
-  if (e == hipSuccess && proj_lds_bytes(cnt, N) > 65536)
-    e = hipFuncSetAttribute((const void *)k_proj_coef,
-                            hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)proj_lds_bytes(cnt, N));
+  if (e == hipSuccess) e = coef_lds_attr(cnt, N);
   if (e == hipSuccess)
     e = hipMemcpy(u, g_new, (size_t)N * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess)
@@ -336,7 +406,7 @@ int qdeq_projection_fit(qdeq_ctx *c, const double *g_new, const double *g_old,
     sl.cnt = cnt;
     for (int i = 0; i < cnt; i++) sl.s[i] = i;
     // the new column as a one-column field (pitch 1)
-    hipLaunchKernelGGL(k_proj_coef, dim3(1), dim3(PROJ_THREADS),
+    hipLaunchKernelGGL(coef_kernel(cnt), dim3(1), dim3(PROJ_THREADS),
                        proj_lds_bytes(cnt, N), c->stream, u, N, 1, g, sl, cf);
     e = hipGetLastError();
   }

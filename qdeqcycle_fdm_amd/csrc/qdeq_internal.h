@@ -108,6 +108,7 @@ struct qdeq_ctx {
   hipStream_t stream = nullptr;
   hipGraphExec_t g_cycle = nullptr;   // one V-cycle (or IR sweep) + residual norm
   int g_cycle_mode = -1;
+  bool direct_launch = false;  // launch V-cycles kernel by kernel (profiling)
   std::string err;
   // RK43 host state (odeSolver.cpp:1338-1363)
   bool rk_ready = false;

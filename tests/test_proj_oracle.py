@@ -44,7 +44,9 @@ def test_exact_span_is_reproduced():
     c, ratio, kept = proj.fit(g_new, g_old)
     assert kept == 4
     np.testing.assert_allclose(c, w, atol=1e-8)
-    assert ratio < 1e-9
+    # the misfit norm comes from a Cholesky pivot: about sqrt(eps) of ||d||
+    # when d lies in the span
+    assert ratio < 1e-6
 
 
 def test_dependent_column_is_left_out():
@@ -73,9 +75,9 @@ def test_block_sum_order_on_exact_sums():
     assert proj.block_sum(v) == 1025 * 1026 / 2
     # one 1.0 and 1023 terms of 1e-16: summed left to right every small term
     # is lost against 1.0 (the result is exactly 1.0); in the kernel's order
-    # threads 1..255 first sum their four small terms apart from the 1.0
-    # (thread 0 loses its three), so the result is 1 + 1020e-16 to within a
-    # few units in the last place
+    # threads 1..255 first sum their four small terms, and the pairwise tree
+    # adds those sums to each other before they meet the 1.0, so the result
+    # is 1 + 1020e-16 to within a few units in the last place
     v = np.full(1024, 1e-16)
     v[0] = 1.0
     left_to_right = 0.0
