@@ -212,6 +212,52 @@ def run_learned(pb, me, g, warmup, tmax):
     return eng, out
 
 
+def run_projection(me, warmup, tmax, slots, tol):
+    """The fast FP64 run from t = 0 with the projection start (`slots`
+    stored solutions, 0 = the plain start) at relative residual `tol`:
+    warm-up, then timed until t = tmax (the end of the plain run's
+    comparison window).  Returns the rates and the final fault state."""
+    from qdeqcycle_fdm_amd import config
+    pb = config.make_bp1_problem(1025, basin=True, tol_mode=config.TOL_REL, tol=tol)
+    eng = make_engine(pb, me, "fast64", tmax=tmax, max_vcycles=200)
+    if slots:
+        eng.projection_enable(slots)
+    eng.rk43_setup()
+    if warmup:
+        eng.rk43_step(warmup)
+    sync()
+    i0 = eng.state()["info"]
+    s0 = eng.projection_stats()
+    t0 = time.perf_counter()
+    info = eng.rk43_step(10 ** 7)
+    sync()
+    t1 = time.perf_counter()
+    s1 = eng.projection_stats()
+    st = eng.state()
+    eng.close()
+    steps = info["it"] - i0["it"]
+    solves = info["solves"] - i0["solves"]
+    out = dict(slots=slots, tol=tol, steps=steps, seconds=round(t1 - t0, 3),
+               steps_per_s=steps / (t1 - t0),
+               vcycles_per_solve=(info["vcycles"] - i0["vcycles"]) / max(solves, 1),
+               solves_per_step=solves / max(steps, 1), t_start=i0["t"], t_end=info["t"])
+    if slots:
+        out["offered"] = s1["offered"] - s0["offered"]
+        out["used"] = s1["used"] - s0["used"]
+    return out, st
+
+
+def fault_diff(a, b):
+    """Largest relative slip difference and largest |log10 V| difference on
+    the fault nodes between two states."""
+    import numpy as np
+    sa, sb = a["slip"][1:-1], b["slip"][1:-1]
+    va = np.log10(np.maximum(a["V"][1:-1], 1e-300))
+    vb = np.log10(np.maximum(b["V"][1:-1], 1e-300))
+    return (float(np.max(np.abs(sa - sb)) / np.max(np.abs(sb))),
+            float(np.max(np.abs(va - vb))))
+
+
 def run_ensemble(members, steps, precision_mode):
     """Ensemble workload: `members` simulations (seeded draws of grid
     257/513/1025, Dc, sigma_n, b; basin medium), `steps` accepted steps each,
@@ -415,6 +461,10 @@ def main():
     ap.add_argument("--ensemble-mode", default="ir32", choices=("fast64", "ir32"))
     ap.add_argument("--no-autotune", action="store_true",
                     help="default tile shapes instead of the autotuned ones")
+    ap.add_argument("--no-projection", action="store_true",
+                    help="skip the projection-start runs")
+    ap.add_argument("--proj-slots", type=int, default=8)
+    ap.add_argument("--proj-tol", type=float, default=1e-12)
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the learned-guess harvest, training and run")
     ap.add_argument("--harvest-runs", type=int, default=2)
@@ -480,7 +530,8 @@ def main():
     # The learned-vs-plain window ends in the quiet phase after the first
     # event (step 2800 of the default run), where both runs can be stopped
     # at the same time; the plain run's state there is copied out untimed.
-    split = args.window_steps if args.mode == "fast64" and not args.no_learned else None
+    split = (args.window_steps if args.mode == "fast64" and
+             not (args.no_learned and args.no_projection) else None)
     eng, main_run = run_mode(pb, me, args.mode, args.warmup, args.steps, timed,
                              split_at=split)
     log(f"{args.mode}: {main_run['steps_per_s']:.2f} steps/s, "
@@ -515,6 +566,41 @@ def main():
         learned["ends_at_window_end"] = lr["t_end"] == win["t_end"]
         learned["slip_max_rel_diff_vs_plain_at_window_end"] = dslip
         learned["training"] = tinfo
+    # Projection start against the plain start over the same window, and the
+    # accuracy of both against the plain start at a 100x tighter residual
+    projection = None
+    if not args.no_projection and args.mode == "fast64" and "window" in main_run:
+        win = main_run["window"]
+        t_end = win["t_end"]
+        ref, st_ref = run_projection(me, args.warmup, t_end, 0, 1e-12)
+        ds, dv = fault_diff(main_run["window_state"], st_ref)
+        log(f"plain 1e-12 over the window: {ref['steps_per_s']:.2f} steps/s, "
+            f"{ref['vcycles_per_solve']:.3f} V-cycles/solve; plain 1e-10 vs it: "
+            f"slip {ds:.2e}, log10 V {dv:.2e}")
+        projection = {
+            "window": [win["t_start"], t_end],
+            "plain_1e-10": {"steps_per_s": round(win["steps_per_s"], 2),
+                            "vcycles_per_solve": round(win["vcycles_per_solve"], 3),
+                            "slip_max_rel_diff_vs_plain_1e-12": ds,
+                            "log10V_max_abs_diff_vs_plain_1e-12": dv},
+            "plain_1e-12": {"steps_per_s": round(ref["steps_per_s"], 2),
+                            "vcycles_per_solve": round(ref["vcycles_per_solve"], 3)},
+            "runs": []}
+        for slots, tol in ((args.proj_slots, args.proj_tol), (6, 1e-10)):
+            pr, st = run_projection(me, args.warmup, t_end, slots, tol)
+            ds, dv = fault_diff(st, st_ref)
+            rec = {k: (round(v, 4) if isinstance(v, float) and k not in ("tol", "t_start", "t_end") else v)
+                   for k, v in pr.items()}
+            rec["ends_at_window_end"] = pr["t_end"] == t_end
+            rec["speedup_vs_plain_1e-10"] = round(pr["steps_per_s"] / win["steps_per_s"], 3)
+            rec["slip_max_rel_diff_vs_plain_1e-12"] = ds
+            rec["log10V_max_abs_diff_vs_plain_1e-12"] = dv
+            projection["runs"].append(rec)
+            log(f"projection {slots} slots, residual {tol:g}: {pr['steps_per_s']:.2f} "
+                f"steps/s ({rec['speedup_vs_plain_1e-10']:.2f}x plain 1e-10), "
+                f"{pr['vcycles_per_solve']:.3f} V-cycles/solve, used "
+                f"{pr.get('used')}/{pr.get('offered')}; vs plain 1e-12: slip {ds:.2e}, "
+                f"log10 V {dv:.2e}")
     main_run.pop("window_state", None)
 
     # per-kernel durations of the tiled cycle (each kernel launched back to
@@ -613,6 +699,10 @@ def main():
         "plain": {k: main_run[k] for k in ("steps", "steps_per_s",
                                             "vcycles_per_solve", "solves_per_step")},
         "learned": learned,
+        "vcycles_per_solve_projection": (
+            None if projection is None else
+            round(projection["runs"][0]["vcycles_per_solve"], 3)),
+        "projection": projection,
         "roofline": roofline,
         "kernel_breakdown_us": breakdown,
         "cpu_baseline": cpu,
