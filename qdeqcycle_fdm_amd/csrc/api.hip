@@ -192,7 +192,7 @@ int qdeq_create(const qdeq_config *cfg, const double *mu,
                            hipHostMallocMapped | hipHostMallocCoherent));
   CREATE_HIP(hipHostGetDevicePointer((void **)&c->mbox_dev, c->mbox_host, 0));
   std::memset(c->mbox_host, 0, SC_COUNT * sizeof(double));
-  CREATE_HIP(alloc_zero(&c->gate, 4, c->stream));
+  CREATE_HIP(alloc_zero(&c->gate, 8, c->stream));
   CREATE_HIP(hipHostMalloc(&c->flags_host, 4 * sizeof(int)));
   CREATE_HIP(alloc_zero(&c->stage_U, (size_t)2 * c->nx * c->nz, c->stream));
   for (auto &p : c->stage_v) CREATE_HIP(alloc_zero(&p, (size_t)c->nz, c->stream));
@@ -241,6 +241,10 @@ int qdeq_create(const qdeq_config *cfg, const double *mu,
   // cannot trace graph launches
   const char *dl = std::getenv("QDEQ_DIRECT_LAUNCH");
   c->direct_launch = dl && dl[0] == '1';
+  // QDEQ_HOST_START=1: the host reads every solve's start norms and decides
+  // (the path FP32-IR solves always take) instead of the device
+  const char *hs = std::getenv("QDEQ_HOST_START");
+  c->host_start = hs && hs[0] == '1';
   *out = c;
   return QDEQ_OK;
 }

@@ -257,6 +257,139 @@ __global__ __launch_bounds__(NORM_THREADS) void k_norm_gate(
   }
 }
 
+// Start of an FP64 solve decided on the device.  Pass 1: the residual-norm
+// partials of the start u, of a candidate start (when cand is given) and of
+// the boundary data, each over the same decomposition and in the same order
+// as k_resnorm_partial and k_bdres_partial (8 blocks), so every norm has the
+// same bits as when those kernels compute it.  partial[0..NORM_BLOCKS): u;
+// partial[NORM_BLOCKS..+8): boundary data; partial[2*NORM_BLOCKS..): cand.
+constexpr int BDRES_BLOCKS = 8;
+
+__global__ __launch_bounds__(NORM_THREADS) void k_start_partials(
+    const double *__restrict__ u, const double *__restrict__ cand,
+    const double *__restrict__ cx, const double *__restrict__ cz, int N, int K,
+    int P, double DH2, int bd, double *__restrict__ partial) {
+  __shared__ double sh[NORM_THREADS];
+  const int W = K - 2;
+  const long total = (long)N * W;
+  double acc = 0.0, accc = 0.0;
+  for (long p = (long)blockIdx.x * NORM_THREADS + threadIdx.x; p < total;
+       p += (long)gridDim.x * NORM_THREADS) {
+    const int j = (int)(p / W);
+    const int k = (int)(p - (long)j * W) + 1;
+    const int jS = (j == 0) ? 1 : j - 1;
+    const int jN = (j == N - 1) ? N - 2 : j + 1;
+    const size_t c = (size_t)j * P + k;
+    const double m1 = cx[c], m2 = cx[c + 1], m3 = cz[c], m4 = cz[c + P];
+    const double r =
+        0.0 - (m1 * u[c - 1] + m2 * u[c + 1] + m3 * u[(size_t)jS * P + k] +
+               m4 * u[(size_t)jN * P + k] - (m1 + m2 + m3 + m4) * u[c]) /
+                  DH2;
+    acc += r * r;
+    if (cand) {
+      const double rc =
+          0.0 - (m1 * cand[c - 1] + m2 * cand[c + 1] + m3 * cand[(size_t)jS * P + k] +
+                 m4 * cand[(size_t)jN * P + k] - (m1 + m2 + m3 + m4) * cand[c]) /
+                    DH2;
+      accc += rc * rc;
+    }
+  }
+  double s = block_sum_256(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  if (cand) {
+    __syncthreads();
+    s = block_sum_256(accc, sh);
+    if (threadIdx.x == 0) partial[2 * NORM_BLOCKS + blockIdx.x] = s;
+  }
+  if (bd && blockIdx.x < BDRES_BLOCKS) {
+    double accb = 0.0;
+    for (int j = blockIdx.x * NORM_THREADS + threadIdx.x; j < N;
+         j += BDRES_BLOCKS * NORM_THREADS) {
+      const size_t row = (size_t)j * P;
+      for (int pass = 0; pass < 2; pass++) {
+        const int k = pass == 0 ? 1 : K - 2;
+        if (pass == 1 && K - 2 == 1) break;
+        const double uW = (k - 1 == 0) ? u[row] : 0.0;
+        const double uE = (k + 1 == K - 1) ? u[row + K - 1] : 0.0;
+        const double m1 = cx[row + k], m2 = cx[row + k + 1];
+        const double r = 0.0 - (m1 * uW + m2 * uE) / DH2;
+        accb += r * r;
+      }
+    }
+    __syncthreads();
+    s = block_sum_256(accb, sh);
+    if (threadIdx.x == 0) partial[NORM_BLOCKS + blockIdx.x] = s;
+  }
+}
+
+// Pass 2 (one block): the norms (sums as k_norm_final), the start (the
+// candidate when it is finite and below the plain start's norm, the rule of
+// the host path), the threshold and the gate of the first batch: gate[0]
+// stop, gate[1] = 0 V-cycles, gate[2] outcome (1 when the start already
+// meets the threshold, 3 when no V-cycle is allowed), gate[4] candidate
+// used.  Counts as one step of the mailbox sequence.
+__global__ __launch_bounds__(NORM_THREADS) void k_start_gate(
+    const double *__restrict__ partial, int cand, int rel, double tol, int cap,
+    double *__restrict__ scal, double *__restrict__ mb, int *__restrict__ gate) {
+  __shared__ double sh[NORM_THREADS];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < NORM_BLOCKS; i += NORM_THREADS) a += partial[i];
+  const double su = block_sum_256(a, sh);
+  __syncthreads();
+  double b = 0.0;
+  if (rel)
+    for (int i = threadIdx.x; i < BDRES_BLOCKS; i += NORM_THREADS)
+      b += partial[NORM_BLOCKS + i];
+  const double sb = block_sum_256(b, sh);
+  __syncthreads();
+  double cc = 0.0;
+  if (cand)
+    for (int i = threadIdx.x; i < NORM_BLOCKS; i += NORM_THREADS)
+      cc += partial[2 * NORM_BLOCKS + i];
+  const double sc = block_sum_256(cc, sh);
+  if (threadIdx.x == 0) {
+    const double du = sqrt(su), dc = sqrt(sc);
+    const double r0 = rel ? sqrt(sb) : du;
+    const bool use = cand && !(isnan(dc) || isinf(dc)) && dc < du;
+    const double d = use ? dc : du;
+    const double thr = rel ? tol * r0 : tol;
+    int code = 0;
+    if (!(d > thr))
+      code = 1;
+    else if (cap < 1)
+      code = 3;
+    scal[SC_NORM] = d;
+    scal[SC_NORM0] = r0;
+    gate[0] = code ? 1 : 0;
+    gate[1] = 0;
+    gate[2] = code;
+    gate[4] = use ? 1 : 0;
+    mb[SC_NORM] = d;
+    mb[SC_NORM0] = r0;
+    if (cand) {
+      scal[SC_NORM_PRED] = dc;
+      mb[SC_NORM_PRED] = dc;
+    }
+    mb[MB_START] = d;
+    mb[MB_USED] = use ? 1.0 : 0.0;
+    mb[MB_COUNT] = 0.0;
+    mb[MB_CODE] = (double)code;
+    const int seq = gate[3] + 1;
+    gate[3] = seq;
+    __threadfence_system();
+    mb[MB_SEQ] = (double)seq;
+  }
+}
+
+// dst = src when flag[0] (the candidate start was chosen).
+__global__ void k_copy_if(double *__restrict__ dst, const double *__restrict__ src,
+                          size_t n, const int *__restrict__ flag) {
+  if (!flag[0]) return;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 // IR update u += e (interior columns), FP64 += FP32.
 __global__ __launch_bounds__(BX *BY) void k_add_correction(
     double *__restrict__ u, const float *__restrict__ e, int N, int K, int P) {
@@ -434,6 +567,13 @@ static void launch_bdres(qdeq_ctx *c) {
                      c->mbox_dev + SC_NORM0);
 }
 
+// Whether solves of this context decide their start on the device: FP64
+// tiled solves, unless QDEQ_HOST_START=1.  The choices and norms are the
+// same either way; the host then waits only for the V-cycles.
+bool device_start(const qdeq_ctx *c) {
+  return !c->host_start && c->cfg.precision == QDEQ_PREC_FP64 && use_tiled(c);
+}
+
 // Whether this context runs the fast cycle with the tiled kernels.
 bool use_tiled(const qdeq_ctx *c) {
   return c->cfg.cycle == QDEQ_CYCLE_FAST && c->tiled &&
@@ -574,6 +714,8 @@ static int mbox_check(qdeq_ctx *c, int slot, bool block, int *ready) {
 // arrives and enqueues the next V-cycle while ||r||_2 > threshold, without
 // blocking unless asked to.  A solve runs the same launches in the same order
 // whether it is polled or waited for.
+static int launch_batch(qdeq_ctx *c, int k);
+
 int mg_begin(qdeq_ctx *c) {
   int st = ensure_graph(c);
   if (st != QDEQ_OK) return st;
@@ -586,6 +728,33 @@ int mg_begin(qdeq_ctx *c) {
   c->cand_pending = false;
   m.it = 0;
   m.sample = false;
+  m.dev_start = device_start(c);
+  m.start_read = false;
+  if (m.dev_start) {
+    // norms, start choice and stop test on the device; the first batch is
+    // launched at once, sized from the last solve
+    const LevelDims &d = c->lv[0];
+    const int rel = c->cfg.tol_mode == QDEQ_TOL_REL ? 1 : 0;
+    hipLaunchKernelGGL(k_start_partials, dim3(NORM_BLOCKS), dim3(NORM_THREADS), 0,
+                       c->stream, c->d[0].u, m.cand ? c->d[0].r : nullptr,
+                       c->d[0].cx, c->d[0].cz, d.N, d.K, d.P, d.DH * d.DH, rel,
+                       c->partial);
+    hipLaunchKernelGGL(k_start_gate, dim3(1), dim3(NORM_THREADS), 0, c->stream,
+                       c->partial, m.cand ? 1 : 0, rel, c->cfg.tol,
+                       c->cfg.max_vcycles, c->scal, c->mbox_dev, c->gate);
+    if (m.cand) {
+      const size_t n = (size_t)d.N * d.P;
+      const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+      hipLaunchKernelGGL(k_copy_if, dim3(blocks), dim3(256), 0, c->stream,
+                         c->d[0].u, c->d[0].r, n, c->gate + 4);
+    }
+    c->seq_issued += 1;
+    QDEQ_HIP(c, hipGetLastError());
+    int st = launch_batch(c, std::min(std::max(c->last_it, 1), 6));
+    if (st != QDEQ_OK) return st;
+    m.phase = 2;
+    return QDEQ_OK;
+  }
   mbox_arm(c, SC_NORM);
   mbox_arm(c, SC_NORM0);
   if (c->cfg.tol_mode == QDEQ_TOL_REL) launch_bdres(c);
@@ -598,6 +767,7 @@ int mg_begin(qdeq_ctx *c) {
 static int mg_finish(qdeq_ctx *c, int st) {
   MgSolve &m = c->mg;
   c->vcycles += m.it;
+  c->last_it = m.it;
   if (m.ir) c->ir_sweeps += m.it;
   c->solves += 1;
   // contraction per V-cycle seen by this solve (for the next batch sizes)
@@ -740,6 +910,19 @@ int mg_poll(qdeq_ctx *c, bool block, int *done) {
         c->timed_calls += 1;
       }
       m.sample = false;
+    }
+    if (m.dev_start && !m.start_read) {
+      // the start's norms, written before the first sequence step
+      const double r0 = c->mbox_host[SC_NORM0];
+      m.delta0 = c->mbox_host[MB_START];
+      m.r0 = c->cfg.tol_mode == QDEQ_TOL_REL ? r0 : m.delta0;
+      m.thr = c->cfg.tol_mode == QDEQ_TOL_REL ? c->cfg.tol * r0 : c->cfg.tol;
+      c->scal_host[SC_NORM0] = r0;
+      if (m.cand) {
+        c->scal_host[SC_NORM_PRED] = c->mbox_host[SC_NORM_PRED];
+        if (c->mbox_host[MB_USED] != 0.0) c->g_accepted++;
+      }
+      m.start_read = true;
     }
     const int code = (int)c->mbox_host[MB_CODE];
     m.it = (int)c->mbox_host[MB_COUNT];

@@ -314,7 +314,8 @@ int proj_before_solve(qdeq_ctx *c) {
   if (c->proj_slots < 1 || c->proj_count < 1) return QDEQ_OK;
   const LevelDims &d = c->lv[0];
   const ProjSlots sl = proj_order(c);
-  mbox_arm(c, SC_NORM_PRED);
+  const bool dev = device_start(c);  // the start's norms come with the solve
+  if (!dev) mbox_arm(c, SC_NORM_PRED);
   if (sl.cnt >= 2)
     hipLaunchKernelGGL(coef_kernel(sl.cnt), dim3(1), dim3(PROJ_THREADS),
                        proj_lds_bytes(sl.cnt, d.N), c->stream, c->d[0].u, d.N, d.P,
@@ -323,7 +324,7 @@ int proj_before_solve(qdeq_ctx *c) {
   hipLaunchKernelGGL(k_proj_apply, grd, blk, 0, c->stream, c->d[0].u, c->proj_wr,
                      c->proj_z, (size_t)d.N * d.P, sl, c->proj_coef, d.N, d.K, d.P,
                      c->d[0].r);
-  launch_resnorm_of(c, c->d[0].r, 2 * 1024, SC_NORM_PRED);
+  if (!dev) launch_resnorm_of(c, c->d[0].r, 2 * 1024, SC_NORM_PRED);
   QDEQ_HIP(c, hipGetLastError());
   c->cand_pending = true;
   c->g_predicted++;

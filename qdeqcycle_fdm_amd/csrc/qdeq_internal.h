@@ -58,10 +58,12 @@ enum {
   SC_TOTERR = 2,
   SC_VMAX = 3,
   SC_TSMAX = 4,
-  SC_NORM_PRED = 5,  // residual norm of the learned-guess candidate
+  SC_NORM_PRED = 5,  // residual norm of the candidate start
   MB_SEQ = 8,        // mailbox only: sequence number of the last V-cycle seen
   MB_COUNT = 9,      //   V-cycles done in the current solve
   MB_CODE = 10,      //   outcome of the last V-cycle (k_norm_gate)
+  MB_USED = 11,      //   1 if the solve started from the candidate
+  MB_START = 12,     //   residual norm of the start used
   SC_COUNT = 16
 };
 
@@ -71,6 +73,8 @@ struct OutputRing;  // output.hip
 struct MgSolve {
   int phase = 0;        // 0 idle, 1 initial norms pending, 2 V-cycles pending
   bool ir = false, cand = false, sample = false;
+  // start decided on the device (k_start_gate); its norms not yet read back
+  bool dev_start = false, start_read = false;
   int it = 0;
   double delta = 0, delta0 = 0, r0 = 0, thr = 0;
   long long seq_target = 0;  // sequence number ending the launched batch
@@ -109,6 +113,8 @@ struct qdeq_ctx {
   hipGraphExec_t g_cycle = nullptr;   // one V-cycle (or IR sweep) + residual norm
   int g_cycle_mode = -1;
   bool direct_launch = false;  // launch V-cycles kernel by kernel (profiling)
+  bool host_start = false;     // start decisions on the host (see mg_begin)
+  int last_it = 2;             // V-cycles of the last solve (first batch size)
   std::string err;
   // RK43 host state (odeSolver.cpp:1338-1363)
   bool rk_ready = false;
@@ -192,6 +198,7 @@ int mg_poll(qdeq_ctx *c, bool block, int *done);
 int mg_build_coefficients(qdeq_ctx *c);
 void mg_destroy_graphs(qdeq_ctx *c);
 bool use_tiled(const qdeq_ctx *c);
+bool device_start(const qdeq_ctx *c);
 
 void launch_resnorm_of(qdeq_ctx *c, const double *u, int off, int slot);
 void mbox_arm(qdeq_ctx *c, int slot);

@@ -110,3 +110,29 @@ def test_projection_and_learned_guess_are_exclusive():
             eng.attach_guess(None, lib.GUESS_OFF)
     finally:
         g.close()
+
+
+@pytest.mark.parametrize("slots", [0, 6])
+def test_device_start_equals_host_start(slots, monkeypatch):
+    """FP64 solves decide their start on the device (norms, candidate
+    choice, stop test) and launch the first V-cycles without waiting;
+    QDEQ_HOST_START=1 restores the host decision.  Same norms, same choices:
+    the runs are bit for bit equal."""
+    runs = []
+    for host in (False, True):
+        if host:
+            monkeypatch.setenv("QDEQ_HOST_START", "1")
+        else:
+            monkeypatch.delenv("QDEQ_HOST_START", raising=False)
+        with _engine(129, tol=1e-10) as eng:
+            if slots:
+                eng.projection_enable(slots)
+            eng.rk43_setup()
+            eng.rk43_step(80)
+            runs.append((eng.state(), eng.projection_stats()))
+    (a, pa), (b, pb_) = runs
+    assert a["info"]["vcycles"] == b["info"]["vcycles"]
+    assert a["info"]["solves"] == b["info"]["solves"]
+    assert pa == pb_
+    for k in ("slip", "psi", "V", "U"):
+        np.testing.assert_array_equal(a[k], b[k])
