@@ -618,6 +618,7 @@ int qdeq_rk43_setup(qdeq_ctx *c, const double *a, const double *b,
   c->vcycles = c->solves = c->ir_sweeps = 0;
   c->rk_ready = false;
   c->rk.phase = 0;  // idle
+  c->rk.staged = false;
   c->mg.phase = 0;
   guess_reset(c);
   proj_reset(c);
@@ -712,7 +713,8 @@ static int rk_advance(qdeq_ctx *c, bool block, int *done) {
         r.phase = RK_STAGE_BEGIN;
         break;
       case RK_STAGE_BEGIN:  // boundary data, then the elliptic solve
-        launch_stage(c, r.s, c->t, c->dt);
+        if (!r.staged) launch_stage(c, r.s, c->t, c->dt);
+        r.staged = false;
         if ((st = guess_before_solve(c))) return st;
         if ((st = proj_before_solve(c))) return st;
         if ((st = mg_begin(c))) return st;
@@ -734,6 +736,9 @@ static int rk_advance(qdeq_ctx *c, bool block, int *done) {
             c->timed_ms += ms;
             c->timed_calls += 1;
           }
+        } else if (c->rk_ready && !r.final_stage && r.s < 6) {
+          launch_update_stage(c, r.s, c->t, c->dt);  // and stage s + 1
+          r.staged = true;
         } else {
           launch_fault_update(c, r.s);
         }

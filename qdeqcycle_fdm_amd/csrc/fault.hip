@@ -170,13 +170,12 @@ __global__ void k_dpsi(const double *psi, const double *V, const double *b,
 // its boundary data written into the compact level-0 u (setBoundary with
 // ratio c_s: fault column slip/2, remote column Vini (t + c_s dt)/2 +
 // SlipShift).  s = 1 is the committed state (odeSolver.cpp:1615).
-__global__ void k_stage(int s, double dt, double t, double ratio, double Vini,
-                        const double *slip, const double *psi,
-                        const double *ss, double *const *FV,
-                        double *const *Fd, double *Ys, double *Yp, double *u,
-                        int nk, int K, int P) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  if (k > nk) return;
+__device__ inline void stage_node(int k, int s, double dt, double t, double ratio,
+                                  double Vini, const double *slip,
+                                  const double *psi, const double *ss,
+                                  double *const *FV, double *const *Fd,
+                                  double *Ys, double *Yp, double *u, int K,
+                                  int P) {
   double ys = slip[k], yp = psi[k];
   if (s >= 2) {
     ys = slip[k] + kA[s][1] * dt * FV[1][k];
@@ -193,18 +192,26 @@ __global__ void k_stage(int s, double dt, double t, double ratio, double Vini,
   row[K - 1] = 0.50 * Vini * (t + ratio * dt) + ss[k];
 }
 
+__global__ void k_stage(int s, double dt, double t, double ratio, double Vini,
+                        const double *slip, const double *psi,
+                        const double *ss, double *const *FV,
+                        double *const *Fd, double *Ys, double *Yp, double *u,
+                        int nk, int K, int P) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  if (k > nk) return;
+  stage_node(k, s, dt, t, ratio, Vini, slip, psi, ss, FV, Fd, Ys, Yp, u, K, P);
+}
+
 // Fault update after a stage solve: Ts from the compact u (columns 0..3 are
 // the fault and its three neighbours), NR for V from the stage's previous
 // rate (zeroed, odeSolver.cpp:1484-1488), then dpsi.
-__global__ void k_fault_update(const double *u, int P, int nk,
-                               const double *mu_f, const double *tau_n0,
-                               const double *Yp, const double *a,
-                               const double *b, const double *eta, double V0,
-                               double f0, double L, double c0, double c1,
-                               double c2, double c3, double *Ts, double *Tn,
-                               double *V, double *dpsi, int *flags) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
-  if (k > nk) return;
+__device__ inline void update_node(int k, const double *u, int P,
+                                   const double *mu_f, const double *tau_n0,
+                                   const double *Yp, const double *a,
+                                   const double *b, const double *eta, double V0,
+                                   double f0, double L, double c0, double c1,
+                                   double c2, double c3, double *Ts, double *Tn,
+                                   double *V, double *dpsi, int *flags) {
   const double *row = u + (size_t)(k - 1) * P;
   const double du = c0 * row[0] + c1 * row[1] + c2 * row[2] + c3 * row[3];
   const double ts = mu_f[k] * du * 1.0e3;
@@ -219,6 +226,49 @@ __global__ void k_fault_update(const double *u, int P, int nk,
   }
   V[k] = Vn;
   dpsi[k] = dpsi_node(psi, Vn, b[k], f0, V0, L);
+}
+
+__global__ void k_fault_update(const double *u, int P, int nk,
+                               const double *mu_f, const double *tau_n0,
+                               const double *Yp, const double *a,
+                               const double *b, const double *eta, double V0,
+                               double f0, double L, double c0, double c1,
+                               double c2, double c3, double *Ts, double *Tn,
+                               double *V, double *dpsi, int *flags) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  if (k > nk) return;
+  update_node(k, u, P, mu_f, tau_n0, Yp, a, b, eta, V0, f0, L, c0, c1, c2, c3, Ts,
+              Tn, V, dpsi, flags);
+}
+
+// The fault update after the solve of stage s and the stage state and
+// boundary data of stage s + 1, in one launch: both are per node (node k
+// reads and writes only its own row of u and its own entries), so each
+// thread runs the two in order.
+struct UpdateArgs {
+  const double *mu_f, *tau_n0, *Yp, *a, *b, *eta;
+  double V0, f0, L, c0, c1, c2, c3;
+  double *Ts, *Tn, *V, *dpsi;
+  int *flags;
+};
+struct StageArgs {
+  int s;
+  double dt, t, ratio, Vini;
+  const double *slip, *psi, *ss;
+  double *const *FV;
+  double *const *Fd;
+  double *Ys, *Yp;
+};
+
+__global__ void k_update_stage(double *u, int nk, int K, int P, UpdateArgs ua,
+                               StageArgs sa) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  if (k > nk) return;
+  update_node(k, u, P, ua.mu_f, ua.tau_n0, ua.Yp, ua.a, ua.b, ua.eta, ua.V0, ua.f0,
+              ua.L, ua.c0, ua.c1, ua.c2, ua.c3, ua.Ts, ua.Tn, ua.V, ua.dpsi,
+              ua.flags);
+  stage_node(k, sa.s, sa.dt, sa.t, sa.ratio, sa.Vini, sa.slip, sa.psi, sa.ss, sa.FV,
+             sa.Fd, sa.Ys, sa.Yp, u, K, P);
 }
 
 __global__ void k_zero_rates(double *const *FV, double *const *Fd, int nz) {
@@ -399,6 +449,18 @@ void launch_fault_update(qdeq_ctx *c, int s) {
                      f.tau_n0, f.Yp[s], f.a, f.b, f.eta, c->cfg.V0, c->cfg.f0,
                      c->cfg.L, c->c_stress[0], c->c_stress[1], c->c_stress[2],
                      c->c_stress[3], f.Ts, f.Tn, f.FV[s], f.Fd[s], c->flags);
+}
+
+void launch_update_stage(qdeq_ctx *c, int s, double t, double dt) {
+  FaultBufs &f = c->fb;
+  const UpdateArgs ua{f.mu_f, f.tau_n0, f.Yp[s], f.a, f.b, f.eta, c->cfg.V0,
+                      c->cfg.f0, c->cfg.L, c->c_stress[0], c->c_stress[1],
+                      c->c_stress[2], c->c_stress[3], f.Ts, f.Tn, f.FV[s], f.Fd[s],
+                      c->flags};
+  const StageArgs sa{s + 1, dt, t, kC[s + 1], c->cfg.Vini, f.slip, f.psi,
+                     f.slip_shift, dev_FV(c), dev_Fd(c), f.Ys[s + 1], f.Yp[s + 1]};
+  hipLaunchKernelGGL(k_update_stage, nodes_grid(c->cfg.nk), dim3(FT), 0, c->stream,
+                     c->d[0].u, c->cfg.nk, c->lv[0].K, c->lv[0].P, ua, sa);
 }
 
 void launch_zero_stage_rates(qdeq_ctx *c) {

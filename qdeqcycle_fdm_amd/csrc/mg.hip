@@ -135,6 +135,18 @@ __device__ inline double block_sum_256(double v, double *sh) {
   return sh[0];
 }
 
+// Residual of the level-0 operator at interior point (j, k), f = 0.
+__device__ inline double residual_at(const double *__restrict__ u,
+                                     const double *__restrict__ cx,
+                                     const double *__restrict__ cz, int j, int jS,
+                                     int jN, int k, int P, double DH2) {
+  const size_t c = (size_t)j * P + k;
+  const double m1 = cx[c], m2 = cx[c + 1], m3 = cz[c], m4 = cz[c + P];
+  return 0.0 - (m1 * u[c - 1] + m2 * u[c + 1] + m3 * u[(size_t)jS * P + k] +
+                m4 * u[(size_t)jN * P + k] - (m1 + m2 + m3 + m4) * u[c]) /
+                   DH2;
+}
+
 // Level-0 residual norm, pass 1: fixed-grid partial sums of r^2 (the
 // reference uses calcuDelta_dev + cublasDnrm2, odeSolver.cpp:693-700).
 // Optionally stores r as FP32 (the IR right-hand side) and zeroes the FP32
@@ -147,25 +159,18 @@ __global__ __launch_bounds__(NORM_THREADS) void k_resnorm_partial(
     float *__restrict__ e32, const int *__restrict__ gate) {
   if (gate && gate[0]) return;  // batched solve already converged
   __shared__ double sh[NORM_THREADS];
-  const int W = K - 2;
-  const long total = (long)N * W;
   double acc = 0.0;
-  for (long p = (long)blockIdx.x * NORM_THREADS + threadIdx.x; p < total;
-       p += (long)gridDim.x * NORM_THREADS) {
-    const int j = (int)(p / W);
-    const int k = (int)(p - (long)j * W) + 1;
+  for (int j = blockIdx.x; j < N; j += gridDim.x) {
     const int jS = (j == 0) ? 1 : j - 1;
     const int jN = (j == N - 1) ? N - 2 : j + 1;
-    const size_t c = (size_t)j * P + k;
-    const double m1 = cx[c], m2 = cx[c + 1], m3 = cz[c], m4 = cz[c + P];
-    const double r =
-        0.0 - (m1 * u[c - 1] + m2 * u[c + 1] + m3 * u[(size_t)jS * P + k] +
-               m4 * u[(size_t)jN * P + k] - (m1 + m2 + m3 + m4) * u[c]) /
-                  DH2;
-    acc += r * r;
-    if (STORE32) {
-      r32[c] = (float)r;
-      e32[c] = 0.0f;
+    for (int k = 1 + threadIdx.x; k <= K - 2; k += NORM_THREADS) {
+      const double r = residual_at(u, cx, cz, j, jS, jN, k, P, DH2);
+      acc += r * r;
+      if (STORE32) {
+        const size_t c = (size_t)j * P + k;
+        r32[c] = (float)r;
+        e32[c] = 0.0f;
+      }
     }
   }
   const double s = block_sum_256(acc, sh);
@@ -270,28 +275,17 @@ __global__ __launch_bounds__(NORM_THREADS) void k_start_partials(
     const double *__restrict__ cx, const double *__restrict__ cz, int N, int K,
     int P, double DH2, int bd, double *__restrict__ partial) {
   __shared__ double sh[NORM_THREADS];
-  const int W = K - 2;
-  const long total = (long)N * W;
   double acc = 0.0, accc = 0.0;
-  for (long p = (long)blockIdx.x * NORM_THREADS + threadIdx.x; p < total;
-       p += (long)gridDim.x * NORM_THREADS) {
-    const int j = (int)(p / W);
-    const int k = (int)(p - (long)j * W) + 1;
+  for (int j = blockIdx.x; j < N; j += gridDim.x) {
     const int jS = (j == 0) ? 1 : j - 1;
     const int jN = (j == N - 1) ? N - 2 : j + 1;
-    const size_t c = (size_t)j * P + k;
-    const double m1 = cx[c], m2 = cx[c + 1], m3 = cz[c], m4 = cz[c + P];
-    const double r =
-        0.0 - (m1 * u[c - 1] + m2 * u[c + 1] + m3 * u[(size_t)jS * P + k] +
-               m4 * u[(size_t)jN * P + k] - (m1 + m2 + m3 + m4) * u[c]) /
-                  DH2;
-    acc += r * r;
-    if (cand) {
-      const double rc =
-          0.0 - (m1 * cand[c - 1] + m2 * cand[c + 1] + m3 * cand[(size_t)jS * P + k] +
-                 m4 * cand[(size_t)jN * P + k] - (m1 + m2 + m3 + m4) * cand[c]) /
-                    DH2;
-      accc += rc * rc;
+    for (int k = 1 + threadIdx.x; k <= K - 2; k += NORM_THREADS) {
+      const double r = residual_at(u, cx, cz, j, jS, jN, k, P, DH2);
+      acc += r * r;
+      if (cand) {
+        const double rc = residual_at(cand, cx, cz, j, jS, jN, k, P, DH2);
+        accc += rc * rc;
+      }
     }
   }
   double s = block_sum_256(acc, sh);
@@ -327,7 +321,8 @@ __global__ __launch_bounds__(NORM_THREADS) void k_start_partials(
 // the host path), the threshold and the gate of the first batch: gate[0]
 // stop, gate[1] = 0 V-cycles, gate[2] outcome (1 when the start already
 // meets the threshold, 3 when no V-cycle is allowed), gate[4] candidate
-// used.  Counts as one step of the mailbox sequence.
+// used, gate[5] candidate given but not used.  Counts as one step of the
+// mailbox sequence.
 __global__ __launch_bounds__(NORM_THREADS) void k_start_gate(
     const double *__restrict__ partial, int cand, int rel, double tol, int cap,
     double *__restrict__ scal, double *__restrict__ mb, int *__restrict__ gate) {
@@ -364,6 +359,7 @@ __global__ __launch_bounds__(NORM_THREADS) void k_start_gate(
     gate[1] = 0;
     gate[2] = code;
     gate[4] = use ? 1 : 0;
+    gate[5] = (cand && !use) ? 1 : 0;
     mb[SC_NORM] = d;
     mb[SC_NORM0] = r0;
     if (cand) {
@@ -733,20 +729,27 @@ int mg_begin(qdeq_ctx *c) {
   if (m.dev_start) {
     // norms, start choice and stop test on the device; the first batch is
     // launched at once, sized from the last solve
+    // the candidate is either in u with the plain start saved in r (the
+    // projection start), or in r (the learned guess)
     const LevelDims &d = c->lv[0];
     const int rel = c->cfg.tol_mode == QDEQ_TOL_REL ? 1 : 0;
+    const bool in_u = m.cand && c->cand_in_u;
+    c->cand_in_u = false;
+    const double *plain = in_u ? c->d[0].r : c->d[0].u;
+    const double *cand = !m.cand ? nullptr : (in_u ? c->d[0].u : c->d[0].r);
     hipLaunchKernelGGL(k_start_partials, dim3(NORM_BLOCKS), dim3(NORM_THREADS), 0,
-                       c->stream, c->d[0].u, m.cand ? c->d[0].r : nullptr,
-                       c->d[0].cx, c->d[0].cz, d.N, d.K, d.P, d.DH * d.DH, rel,
-                       c->partial);
+                       c->stream, plain, cand, c->d[0].cx, c->d[0].cz, d.N, d.K, d.P,
+                       d.DH * d.DH, rel, c->partial);
     hipLaunchKernelGGL(k_start_gate, dim3(1), dim3(NORM_THREADS), 0, c->stream,
                        c->partial, m.cand ? 1 : 0, rel, c->cfg.tol,
                        c->cfg.max_vcycles, c->scal, c->mbox_dev, c->gate);
     if (m.cand) {
+      // u <- r: the candidate when it was chosen (in r), or the plain start
+      // back when it was not (candidate in u)
       const size_t n = (size_t)d.N * d.P;
       const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
       hipLaunchKernelGGL(k_copy_if, dim3(blocks), dim3(256), 0, c->stream,
-                         c->d[0].u, c->d[0].r, n, c->gate + 4);
+                         c->d[0].u, c->d[0].r, n, c->gate + (in_u ? 5 : 4));
     }
     c->seq_issued += 1;
     QDEQ_HIP(c, hipGetLastError());

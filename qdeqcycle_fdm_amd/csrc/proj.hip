@@ -222,25 +222,29 @@ __global__ __launch_bounds__(PROJ_THREADS) void k_proj_coef(
 }
 
 // Start on the interior columns, the current U on the fault and remote
-// columns (and the pitch padding).
-__global__ void k_proj_apply(const double *__restrict__ u,
-                             const double *__restrict__ wr,
+// columns (and the pitch padding).  cand may be u itself (each point is read
+// before it is written; the remote value u[K-1] is rewritten unchanged);
+// with backup, u is saved there first.
+__global__ void k_proj_apply(const double *u, const double *__restrict__ wr,
                              const double *__restrict__ z, size_t zstride,
                              ProjSlots sl, const double *__restrict__ coef, int N,
-                             int K, int P, double *__restrict__ cand) {
+                             int K, int P, double *cand, double *__restrict__ backup) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
   if (k >= P || j >= N) return;
   const size_t c = (size_t)j * P + k;
+  const double uc = u[c];
+  if (backup) backup[c] = uc;
   if (k == 0 || k >= K - 1) {
-    cand[c] = u[c];
+    cand[c] = uc;
     return;
   }
+  const double rstar = u[K - 1];
   const double z0 = z[(size_t)sl.s[0] * zstride + c];
   double acc = z0;
   for (int i = 1; i < sl.cnt; i++)
     acc = acc + coef[i] * (z[(size_t)sl.s[i] * zstride + c] - z0);
-  cand[c] = acc + u[K - 1] * wr[c];
+  cand[c] = acc + rstar * wr[c];
 }
 
 // Stores the solution just found: Z = U - r W_R and its fault column.
@@ -321,12 +325,15 @@ int proj_before_solve(qdeq_ctx *c) {
                        proj_lds_bytes(sl.cnt, d.N), c->stream, c->d[0].u, d.N, d.P,
                        c->proj_g, sl, c->proj_coef);
   const dim3 blk(256), grd((d.P + 255) / 256, d.N);
+  // on the device path the start goes straight into u (the plain start is
+  // kept in r); on the host path into r
   hipLaunchKernelGGL(k_proj_apply, grd, blk, 0, c->stream, c->d[0].u, c->proj_wr,
                      c->proj_z, (size_t)d.N * d.P, sl, c->proj_coef, d.N, d.K, d.P,
-                     c->d[0].r);
+                     dev ? c->d[0].u : c->d[0].r, dev ? c->d[0].r : nullptr);
   if (!dev) launch_resnorm_of(c, c->d[0].r, 2 * 1024, SC_NORM_PRED);
   QDEQ_HIP(c, hipGetLastError());
   c->cand_pending = true;
+  c->cand_in_u = dev;
   c->g_predicted++;
   return QDEQ_OK;
 }

@@ -84,6 +84,7 @@ struct MgSolve {
 struct RkMachine {
   int phase = 0;
   int s = 0;               // stage being solved
+  bool staged = false;     // stage s's state already launched (with the update)
   int attempts = 0;        // attempts of the current step
   long long steps_left = 0;
   bool final_stage = false;  // stage 1 after an accepted step
@@ -115,6 +116,7 @@ struct qdeq_ctx {
   bool direct_launch = false;  // launch V-cycles kernel by kernel (profiling)
   bool host_start = false;     // start decisions on the host (see mg_begin)
   int last_it = 2;             // V-cycles of the last solve (first batch size)
+  bool cand_in_u = false;      // the pending candidate start is in u (plain in r)
   std::string err;
   // RK43 host state (odeSolver.cpp:1338-1363)
   bool rk_ready = false;
@@ -304,6 +306,8 @@ void launch_nr(qdeq_ctx *c, const double *Ts, const double *Tn,
 void launch_dpsi(qdeq_ctx *c, const double *psi, const double *V,
                  const double *b, double *dpsi);
 void launch_stage(qdeq_ctx *c, int s, double t, double dt);
+// fault update of stage s and the stage state of s + 1 in one launch
+void launch_update_stage(qdeq_ctx *c, int s, double t, double dt);
 void launch_fault_update(qdeq_ctx *c, int s);
 void launch_zero_stage_rates(qdeq_ctx *c);
 void launch_combine(qdeq_ctx *c, double dt);

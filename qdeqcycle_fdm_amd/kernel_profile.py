@@ -9,6 +9,12 @@ with the shapes and the HIP-event durations.  Does not import torch (its
 bundled HIP runtime would replace the one the library was built against).
 
     python -m qdeqcycle_fdm_amd.kernel_profile --n 1025 --mode fast64 --reps 20
+
+With --rk-steps K it instead runs K accepted RK43 steps (after --rk-warmup
+untimed ones) of the basin workload at residual --tol, with the projection
+start from --proj-slots stored solutions (0: the previous solution), and
+prints the rate.  Under a kernel tracer run it with QDEQ_DIRECT_LAUNCH=1 so
+the V-cycles are launched kernel by kernel instead of as HIP graphs.
 """
 import argparse
 import json
@@ -37,11 +43,42 @@ def main(argv=None):
                          "(as bench.py reports them)")
     ap.add_argument("--copy-doubles", type=int, default=0,
                     help="also time a plain copy of this many doubles")
+    ap.add_argument("--rk-steps", type=int, default=0,
+                    help="run this many accepted RK43 steps instead")
+    ap.add_argument("--rk-warmup", type=int, default=100)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--proj-slots", type=int, default=0)
     args = ap.parse_args(argv)
 
     from qdeqcycle_fdm_amd import config, lib
     pb = config.make_bp1_problem(args.n, basin=True, levels=LEVELS[args.n],
-                                 tol_mode=config.TOL_REL, tol=1e-10)
+                                 tol_mode=config.TOL_REL, tol=args.tol)
+    if args.rk_steps:
+        import time
+        me = lib.build_mu_equi(pb.mu)
+        kw = dict(MODES[args.mode])
+        tiles = kw.pop("tiles")
+        with lib.Engine(pb, me, max_vcycles=200, **kw) as eng:
+            eng.set_tiles(True, *tiles)
+            shapes = eng.autotune_tiles(5) if not args.no_autotune else None
+            if args.proj_slots:
+                eng.projection_enable(args.proj_slots)
+            eng.rk43_setup()
+            eng.rk43_step(args.rk_warmup)
+            lib.device_synchronize(0)
+            i0 = eng.state()["info"]
+            t0 = time.perf_counter()
+            info = eng.rk43_step(args.rk_steps)
+            lib.device_synchronize(0)
+            sec = time.perf_counter() - t0
+            solves = info["solves"] - i0["solves"]
+            out = {"n": args.n, "mode": args.mode, "tol": args.tol,
+                   "proj_slots": args.proj_slots, "tile_shapes": shapes,
+                   "steps": args.rk_steps, "steps_per_s": args.rk_steps / sec,
+                   "solves": solves,
+                   "vcycles_per_solve": (info["vcycles"] - i0["vcycles"]) / max(solves, 1)}
+        print(json.dumps(out), flush=True)
+        return
     me = lib.build_mu_equi(pb.mu)
     kw = dict(MODES[args.mode])
     tiles = kw.pop("tiles")
