@@ -586,7 +586,8 @@ def main():
             "plain_1e-12": {"steps_per_s": round(ref["steps_per_s"], 2),
                             "vcycles_per_solve": round(ref["vcycles_per_solve"], 3)},
             "runs": []}
-        for slots, tol in ((args.proj_slots, args.proj_tol), (6, 1e-10)):
+        for slots, tol in ((args.proj_slots, args.proj_tol), (args.proj_slots, 1e-11),
+                           (6, 1e-10)):
             pr, st = run_projection(me, args.warmup, t_end, slots, tol)
             ds, dv = fault_diff(st, st_ref)
             rec = {k: (round(v, 4) if isinstance(v, float) and k not in ("tol", "t_start", "t_end") else v)
