@@ -138,18 +138,24 @@ def lstsq_reference(g_new: np.ndarray, g_old: np.ndarray) -> np.ndarray:
     return np.concatenate([[0.0], sol])
 
 
-def apply(z: np.ndarray, wr: np.ndarray, u: np.ndarray, c: np.ndarray) -> np.ndarray:
-    """The start field: z (cnt, N, P) stored solutions with the remote part
-    removed (newest first), wr (N, P) the unit remote response, u (N, P) the
-    field as the stage left it (boundary columns hold the new data, the
-    remote value of row 0 at column K-1 = P-1 here), c from ``fit``.
-    Interior columns: z0 + sum_i c_i (z_i - z0) + u[0, K-1] wr, added in that
-    order; columns 0 and K-1 from u."""
+def apply(ring: np.ndarray, rr: np.ndarray, wr: np.ndarray, u: np.ndarray,
+          c: np.ndarray) -> np.ndarray:
+    """The start field: ring (cnt, N, P) the stored solutions (newest
+    first), rr (cnt,) their remote values (row 0 of the remote column), wr
+    (N, P) the response to a unit remote change, u (N, P) the field as the
+    stage left it (boundary columns hold the new data; column K-1 = P-1
+    here is the remote column), c from ``fit``.  Interior columns:
+    U_0 + sum_i c_i (U_i - U_0) + w wr with w = r* - r_0 - sum_i c_i
+    (r_i - r_0), each sum formed in that order; columns 0 and K-1 from u."""
     K = u.shape[1]
-    acc = z[0].copy()
-    for i in range(1, z.shape[0]):
-        acc = acc + c[i] * (z[i] - z[0])
-    out = acc + u[0, K - 1] * wr
+    r0 = rr[0]
+    w = u[0, K - 1] - r0
+    for i in range(1, ring.shape[0]):
+        w = w - c[i] * (rr[i] - r0)
+    acc = ring[0].copy()
+    for i in range(1, ring.shape[0]):
+        acc = acc + c[i] * (ring[i] - ring[0])
+    out = acc + w * wr
     out[:, 0] = u[:, 0]
     out[:, K - 1] = u[:, K - 1]
     return out

@@ -147,6 +147,22 @@ __device__ inline double residual_at(const double *__restrict__ u,
                    DH2;
 }
 
+// The same residual with the interior values from ui and the fault and
+// remote column values from ub (a start whose interior lives elsewhere).
+__device__ inline double residual_at2(const double *__restrict__ ui,
+                                      const double *__restrict__ ub,
+                                      const double *__restrict__ cx,
+                                      const double *__restrict__ cz, int j, int jS,
+                                      int jN, int k, int K, int P, double DH2) {
+  const size_t c = (size_t)j * P + k;
+  const double m1 = cx[c], m2 = cx[c + 1], m3 = cz[c], m4 = cz[c + P];
+  const double uW = k == 1 ? ub[c - 1] : ui[c - 1];
+  const double uE = k == K - 2 ? ub[c + 1] : ui[c + 1];
+  return 0.0 - (m1 * uW + m2 * uE + m3 * ui[(size_t)jS * P + k] +
+                m4 * ui[(size_t)jN * P + k] - (m1 + m2 + m3 + m4) * ui[c]) /
+                   DH2;
+}
+
 // Level-0 residual norm, pass 1: fixed-grid partial sums of r^2 (the
 // reference uses calcuDelta_dev + cublasDnrm2, odeSolver.cpp:693-700).
 // Optionally stores r as FP32 (the IR right-hand side) and zeroes the FP32
@@ -263,24 +279,27 @@ __global__ __launch_bounds__(NORM_THREADS) void k_norm_gate(
 }
 
 // Start of an FP64 solve decided on the device.  Pass 1: the residual-norm
-// partials of the start u, of a candidate start (when cand is given) and of
-// the boundary data, each over the same decomposition and in the same order
-// as k_resnorm_partial and k_bdres_partial (8 blocks), so every norm has the
-// same bits as when those kernels compute it.  partial[0..NORM_BLOCKS): u;
-// partial[NORM_BLOCKS..+8): boundary data; partial[2*NORM_BLOCKS..): cand.
+// partials of the plain start (interior values from ui, fault and remote
+// columns from u), of a candidate start (when cand is given) and of the
+// boundary data, each over the same decomposition and in the same order as
+// k_resnorm_partial and k_bdres_partial (8 blocks), so every norm has the
+// same bits as when those kernels compute it.  partial[0..NORM_BLOCKS):
+// plain start; partial[NORM_BLOCKS..+8): boundary data;
+// partial[2*NORM_BLOCKS..): candidate.
 constexpr int BDRES_BLOCKS = 8;
 
 __global__ __launch_bounds__(NORM_THREADS) void k_start_partials(
-    const double *__restrict__ u, const double *__restrict__ cand,
-    const double *__restrict__ cx, const double *__restrict__ cz, int N, int K,
-    int P, double DH2, int bd, double *__restrict__ partial) {
+    const double *__restrict__ ui, const double *__restrict__ u,
+    const double *__restrict__ cand, const double *__restrict__ cx,
+    const double *__restrict__ cz, int N, int K, int P, double DH2, int bd,
+    double *__restrict__ partial) {
   __shared__ double sh[NORM_THREADS];
   double acc = 0.0, accc = 0.0;
   for (int j = blockIdx.x; j < N; j += gridDim.x) {
     const int jS = (j == 0) ? 1 : j - 1;
     const int jN = (j == N - 1) ? N - 2 : j + 1;
     for (int k = 1 + threadIdx.x; k <= K - 2; k += NORM_THREADS) {
-      const double r = residual_at(u, cx, cz, j, jS, jN, k, P, DH2);
+      const double r = residual_at2(ui, u, cx, cz, j, jS, jN, k, K, P, DH2);
       acc += r * r;
       if (cand) {
         const double rc = residual_at(cand, cx, cz, j, jS, jN, k, P, DH2);
@@ -384,6 +403,16 @@ __global__ void k_copy_if(double *__restrict__ dst, const double *__restrict__ s
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
     dst[i] = src[i];
+}
+
+// u's interior columns (1..K-2) = src's when flag[0] (the plain start
+// restored over a candidate that was not chosen).
+__global__ void k_restore_if(double *__restrict__ u, const double *__restrict__ src,
+                             int N, int K, int P, const int *__restrict__ flag) {
+  if (!flag[0]) return;
+  for (int j = blockIdx.x; j < N; j += gridDim.x)
+    for (int k = 1 + threadIdx.x; k <= K - 2; k += blockDim.x)
+      u[(size_t)j * P + k] = src[(size_t)j * P + k];
 }
 
 // IR update u += e (interior columns), FP64 += FP32.
@@ -729,27 +758,31 @@ int mg_begin(qdeq_ctx *c) {
   if (m.dev_start) {
     // norms, start choice and stop test on the device; the first batch is
     // launched at once, sized from the last solve
-    // the candidate is either in u with the plain start saved in r (the
-    // projection start), or in r (the learned guess)
+    // the candidate is either written over u's interior with the plain
+    // start's interior kept at plain_interior (the projection start), or in
+    // r (the learned guess)
     const LevelDims &d = c->lv[0];
     const int rel = c->cfg.tol_mode == QDEQ_TOL_REL ? 1 : 0;
     const bool in_u = m.cand && c->cand_in_u;
     c->cand_in_u = false;
-    const double *plain = in_u ? c->d[0].r : c->d[0].u;
+    const double *plain = in_u ? c->plain_interior : c->d[0].u;
     const double *cand = !m.cand ? nullptr : (in_u ? c->d[0].u : c->d[0].r);
     hipLaunchKernelGGL(k_start_partials, dim3(NORM_BLOCKS), dim3(NORM_THREADS), 0,
-                       c->stream, plain, cand, c->d[0].cx, c->d[0].cz, d.N, d.K, d.P,
-                       d.DH * d.DH, rel, c->partial);
+                       c->stream, plain, c->d[0].u, cand, c->d[0].cx, c->d[0].cz,
+                       d.N, d.K, d.P, d.DH * d.DH, rel, c->partial);
     hipLaunchKernelGGL(k_start_gate, dim3(1), dim3(NORM_THREADS), 0, c->stream,
                        c->partial, m.cand ? 1 : 0, rel, c->cfg.tol,
                        c->cfg.max_vcycles, c->scal, c->mbox_dev, c->gate);
-    if (m.cand) {
-      // u <- r: the candidate when it was chosen (in r), or the plain start
-      // back when it was not (candidate in u)
+    if (in_u) {
+      // the plain start back over a candidate that was not chosen
+      hipLaunchKernelGGL(k_restore_if, dim3(std::min(d.N, 1024)), dim3(256), 0,
+                         c->stream, c->d[0].u, plain, d.N, d.K, d.P, c->gate + 5);
+    } else if (m.cand) {
+      // the chosen candidate from r into u
       const size_t n = (size_t)d.N * d.P;
-      const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
+      const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 1024);
       hipLaunchKernelGGL(k_copy_if, dim3(blocks), dim3(256), 0, c->stream,
-                         c->d[0].u, c->d[0].r, n, c->gate + (in_u ? 5 : 4));
+                         c->d[0].u, c->d[0].r, n, c->gate + 4);
     }
     c->seq_issued += 1;
     QDEQ_HIP(c, hipGetLastError());

@@ -7,16 +7,19 @@
 // below the fallback's.
 //
 // The elliptic problem has f = 0, so a solution is linear in its boundary
-// data: U = S(g, r) with g the fault column and r the remote value (uniform
-// along the remote column).  With W_R = S(0, 1), Z = U - r W_R = S(g, 0).
-// For new data (g*, r*) the start on the interior columns is
-//     Z_0 + sum_i c_i (Z_i - Z_0) + r* W_R,
-// Z_0 the newest stored Z, where c minimises
+// data: U = S(g, r) with g the fault column and r the remote value (the
+// remote column is r plus a fixed per-row shift).  With W_R = S(0, 1) the
+// response to a unit change of r, the start for new data (g*, r*) on the
+// interior columns is
+//     U_0 + sum_i c_i (U_i - U_0) + w W_R,   w = r* - r_0 - sum_i c_i (r_i - r_0),
+// U_0 the newest stored solution, where c minimises
 //     || (g* - g_0) - sum_i c_i (g_i - g_0) ||_2
 // over the stored fault columns (CholeskyQR applied twice, in one
 // workgroup, every sum in a fixed order).  The start's residual is then
 // the response to the part of g* - g_0 outside the stored span, plus the
-// stored solutions' own residuals.
+// stored solutions' own residuals.  The ring keeps the solutions
+// themselves, so the previous solution (the plain start's interior) is
+// always at hand.
 
 #include <cmath>
 #include <string>
@@ -221,42 +224,44 @@ __global__ __launch_bounds__(PROJ_THREADS) void k_proj_coef(
   }
 }
 
-// Start on the interior columns, the current U on the fault and remote
-// columns (and the pitch padding).  cand may be u itself (each point is read
-// before it is written; the remote value u[K-1] is rewritten unchanged);
-// with backup, u is saved there first.
+// Start on the interior columns (1..K-2) into cand; with copy_bnd the fault
+// and remote columns (and the pitch padding) are copied from u as well.
+// cand may be u itself (interior points only are written then).  The weight
+// w of W_R is formed by every thread alike, in the order above.
 __global__ void k_proj_apply(const double *u, const double *__restrict__ wr,
-                             const double *__restrict__ z, size_t zstride,
-                             ProjSlots sl, const double *__restrict__ coef, int N,
-                             int K, int P, double *cand, double *__restrict__ backup) {
+                             const double *__restrict__ ring, size_t rstride,
+                             const double *__restrict__ rr, ProjSlots sl,
+                             const double *__restrict__ coef, int N, int K, int P,
+                             double *cand, int copy_bnd) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
   if (k >= P || j >= N) return;
   const size_t c = (size_t)j * P + k;
-  const double uc = u[c];
-  if (backup) backup[c] = uc;
   if (k == 0 || k >= K - 1) {
-    cand[c] = uc;
+    if (copy_bnd) cand[c] = u[c];
     return;
   }
-  const double rstar = u[K - 1];
-  const double z0 = z[(size_t)sl.s[0] * zstride + c];
-  double acc = z0;
+  const double r0 = rr[sl.s[0]];
+  double w = u[K - 1] - r0;
+  for (int i = 1; i < sl.cnt; i++) w = w - coef[i] * (rr[sl.s[i]] - r0);
+  const double u0 = ring[(size_t)sl.s[0] * rstride + c];
+  double acc = u0;
   for (int i = 1; i < sl.cnt; i++)
-    acc = acc + coef[i] * (z[(size_t)sl.s[i] * zstride + c] - z0);
-  cand[c] = acc + rstar * wr[c];
+    acc = acc + coef[i] * (ring[(size_t)sl.s[i] * rstride + c] - u0);
+  cand[c] = acc + w * wr[c];
 }
 
-// Stores the solution just found: Z = U - r W_R and its fault column.
-__global__ void k_proj_commit(const double *__restrict__ u,
-                              const double *__restrict__ wr, int N, int K, int P,
-                              double *__restrict__ z, double *__restrict__ g) {
+// Stores the solution just found, its fault column and its remote value.
+__global__ void k_proj_commit(const double *__restrict__ u, int N, int K, int P,
+                              double *__restrict__ dst, double *__restrict__ g,
+                              double *__restrict__ r) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
   if (k >= P || j >= N) return;
   const size_t c = (size_t)j * P + k;
-  z[c] = u[c] - u[K - 1] * wr[c];
+  dst[c] = u[c];
   if (k == 0) g[j] = u[c];
+  if (j == 0 && k == K - 1) r[0] = u[c];
 }
 
 }  // namespace
@@ -299,9 +304,9 @@ void proj_reset(qdeq_ctx *c) {
 }
 
 void proj_free(qdeq_ctx *c) {
-  void *p[] = {c->proj_z, c->proj_g, c->proj_coef, c->proj_wr};
+  void *p[] = {c->proj_u, c->proj_g, c->proj_r, c->proj_coef, c->proj_wr};
   for (void *q : p) (void)hipFree(q);
-  c->proj_z = c->proj_g = c->proj_coef = c->proj_wr = nullptr;
+  c->proj_u = c->proj_g = c->proj_r = c->proj_coef = c->proj_wr = nullptr;
   c->proj_slots = 0;
   proj_reset(c);
 }
@@ -325,15 +330,16 @@ int proj_before_solve(qdeq_ctx *c) {
                        proj_lds_bytes(sl.cnt, d.N), c->stream, c->d[0].u, d.N, d.P,
                        c->proj_g, sl, c->proj_coef);
   const dim3 blk(256), grd((d.P + 255) / 256, d.N);
-  // on the device path the start goes straight into u (the plain start is
-  // kept in r); on the host path into r
+  // on the device path the start is written over u's interior (the plain
+  // start's interior is the newest stored solution); on the host path into r
   hipLaunchKernelGGL(k_proj_apply, grd, blk, 0, c->stream, c->d[0].u, c->proj_wr,
-                     c->proj_z, (size_t)d.N * d.P, sl, c->proj_coef, d.N, d.K, d.P,
-                     dev ? c->d[0].u : c->d[0].r, dev ? c->d[0].r : nullptr);
+                     c->proj_u, (size_t)d.N * d.P, c->proj_r, sl, c->proj_coef,
+                     d.N, d.K, d.P, dev ? c->d[0].u : c->d[0].r, dev ? 0 : 1);
   if (!dev) launch_resnorm_of(c, c->d[0].r, 2 * 1024, SC_NORM_PRED);
   QDEQ_HIP(c, hipGetLastError());
   c->cand_pending = true;
   c->cand_in_u = dev;
+  c->plain_interior = c->proj_u + (size_t)sl.s[0] * d.N * d.P;
   c->g_predicted++;
   return QDEQ_OK;
 }
@@ -343,9 +349,9 @@ int proj_after_solve(qdeq_ctx *c) {
   const LevelDims &d = c->lv[0];
   const int slot = c->proj_head;
   const dim3 blk(256), grd((d.P + 255) / 256, d.N);
-  hipLaunchKernelGGL(k_proj_commit, grd, blk, 0, c->stream, c->d[0].u, c->proj_wr,
-                     d.N, d.K, d.P, c->proj_z + (size_t)slot * d.N * d.P,
-                     c->proj_g + (size_t)slot * d.N);
+  hipLaunchKernelGGL(k_proj_commit, grd, blk, 0, c->stream, c->d[0].u, d.N, d.K, d.P,
+                     c->proj_u + (size_t)slot * d.N * d.P,
+                     c->proj_g + (size_t)slot * d.N, c->proj_r + slot);
   QDEQ_HIP(c, hipGetLastError());
   c->proj_head = (c->proj_head + 1) % c->proj_slots;
   if (c->proj_count < c->proj_slots) c->proj_count++;
@@ -375,8 +381,9 @@ int qdeq_projection_enable(qdeq_ctx *c, int slots) {
                      "projection slots x fault nodes exceed the workgroup LDS");
   proj_free(c);
   const size_t n0 = (size_t)d.N * d.P;
-  QDEQ_HIP(c, hipMalloc(&c->proj_z, (size_t)slots * n0 * sizeof(double)));
+  QDEQ_HIP(c, hipMalloc(&c->proj_u, (size_t)slots * n0 * sizeof(double)));
   QDEQ_HIP(c, hipMalloc(&c->proj_g, (size_t)slots * d.N * sizeof(double)));
+  QDEQ_HIP(c, hipMalloc(&c->proj_r, (size_t)slots * sizeof(double)));
   QDEQ_HIP(c, hipMalloc(&c->proj_coef, (PROJ_MAX_SLOTS + 2) * sizeof(double)));
   QDEQ_HIP(c, hipMalloc(&c->proj_wr, n0 * sizeof(double)));
   QDEQ_HIP(c, hipMemset(c->proj_coef, 0, (PROJ_MAX_SLOTS + 2) * sizeof(double)));

@@ -116,7 +116,10 @@ struct qdeq_ctx {
   bool direct_launch = false;  // launch V-cycles kernel by kernel (profiling)
   bool host_start = false;     // start decisions on the host (see mg_begin)
   int last_it = 2;             // V-cycles of the last solve (first batch size)
-  bool cand_in_u = false;      // the pending candidate start is in u (plain in r)
+  // the pending candidate start is in u's interior; the plain start's
+  // interior is then at plain_interior (same layout as u)
+  bool cand_in_u = false;
+  const double *plain_interior = nullptr;
   std::string err;
   // RK43 host state (odeSolver.cpp:1338-1363)
   bool rk_ready = false;
@@ -181,12 +184,12 @@ struct qdeq_ctx {
   std::chrono::steady_clock::time_point mb_idle_since{};
   long long g_predicted = 0, g_accepted = 0, g_harvested = 0, g_seen = 0;
   hipGraphExec_t g_pred = nullptr;
-  // projection start (proj.hip): ring of proj_slots stored solutions with
-  // the remote part removed (N x P each) and their fault columns, the fit's
-  // coefficients, and W_R
+  // projection start (proj.hip): ring of proj_slots stored solutions (N x P
+  // each), their fault columns and remote values, the fit's coefficients,
+  // and W_R
   int proj_slots = 0, proj_count = 0, proj_head = 0;
-  double *proj_z = nullptr, *proj_g = nullptr, *proj_coef = nullptr,
-         *proj_wr = nullptr;
+  double *proj_u = nullptr, *proj_g = nullptr, *proj_r = nullptr,
+         *proj_coef = nullptr, *proj_wr = nullptr;
 };
 
 namespace qdeq {

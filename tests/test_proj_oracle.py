@@ -89,22 +89,25 @@ def test_block_sum_order_on_exact_sums():
 
 
 def test_start_field_is_exact_for_linear_responses():
-    # fields that depend linearly on their fault column reproduce the new
-    # field exactly when the new column is in the stored span
+    # fields that depend linearly on their fault column and remote value
+    # reproduce the new field exactly when the new column is in the stored
+    # span, whatever the remote values
     rng = np.random.default_rng(11)
     N, K = 65, 33
     prof = np.exp(-np.linspace(0.0, 3.0, K))  # response profile across the grid
     wr = np.tile(np.linspace(0.0, 1.0, K), (N, 1))  # unit remote response
     _, g_old = _columns(rng, 4, N, noise=1.0)
+    r_old = np.array([2.0, 1.9, 1.7, 1.6])
     w = np.array([0.0, 0.3, -0.1, 0.6])
     g_new = g_old[0] + (g_old[1:] - g_old[0]).T @ w[1:]
-    z = np.array([np.outer(g, prof) for g in g_old])
+    ring = np.array([np.outer(g, prof) + r * wr for g, r in zip(g_old, r_old)])
     r_new = 2.5
     u = np.zeros((N, K))
     u[:, 0] = g_new
     u[:, K - 1] = r_new
     c, _, _ = proj.fit(g_new, g_old)
-    start = proj.apply(z, wr, u, c)
+    start = proj.apply(ring, r_old, wr, u, c)
     exact = np.outer(g_new, prof) + r_new * wr
+    exact[:, 0] = g_new
     exact[:, K - 1] = r_new
     np.testing.assert_allclose(start, exact, rtol=0, atol=1e-12)
