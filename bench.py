@@ -258,12 +258,13 @@ def fault_diff(a, b):
             float(np.max(np.abs(va - vb))))
 
 
-def run_ensemble(members, steps, precision_mode):
+def run_ensemble(members, steps, precision_mode, proj_slots=0):
     """Ensemble workload: `members` simulations (seeded draws of grid
     257/513/1025, Dc, sigma_n, b; basin medium), `steps` accepted steps each,
     first one after the other (each alone), then all together with
-    qdeq_ensemble_step.  Returns both rates and whether every member's final
-    state is bit-identical between the two runs."""
+    qdeq_ensemble_step, and with proj_slots > 0 all together again with the
+    projection start.  Returns the rates, V-cycles per solve and whether
+    every member's final state is bit-identical between the first two runs."""
     import numpy as np
     from qdeqcycle_fdm_amd import config, lib
     levels = {257: 5, 513: 6, 1025: 7}
@@ -277,19 +278,25 @@ def run_ensemble(members, steps, precision_mode):
         if d["n"] not in media:
             media[d["n"]] = lib.build_mu_equi(pbs[-1].mu)
 
-    def make(i):
+    def make(i, slots=0):
         pb = pbs[i]
         n = draws[i]["n"]
         kw = dict(MODES[precision_mode])
         tiles = kw.pop("tiles")
-        eng = lib.Engine(pb, media[n], **kw)
+        eng = lib.Engine(pb, media[n], max_vcycles=200, **kw)
         eng.set_tiles(True, *tiles)
         if n not in shapes:
             shapes[n] = eng.autotune_tiles(5)
         else:
             eng.set_tile_shapes(shapes[n])
+        if slots:
+            eng.projection_enable(slots)
         eng.rk43_setup()
         return eng
+
+    def vcycles_per_solve(engines):
+        infos = [e.state()["info"] for e in engines]
+        return sum(i["vcycles"] for i in infos) / max(sum(i["solves"] for i in infos), 1)
 
     alone, t_alone = [], 0.0
     for i in range(members):
@@ -310,14 +317,31 @@ def run_ensemble(members, steps, precision_mode):
     sync()
     t_ens = time.perf_counter() - t0
     same = all(np.array_equal(e.state()["slip"], a) for e, a in zip(engines, alone))
+    vc_plain = vcycles_per_solve(engines)
     for e in engines:
         e.close()
     total = members * steps
-    return dict(members=members, steps_each=steps, mode=precision_mode,
-                grids={str(n): sum(1 for d in draws if d["n"] == n) for n in levels},
-                steps_per_s_alone=total / t_alone, steps_per_s_ensemble=total / t_ens,
-                seconds_alone=t_alone, seconds_ensemble=t_ens,
-                bitwise_equal_to_alone=bool(same))
+    out = dict(members=members, steps_each=steps, mode=precision_mode,
+               grids={str(n): sum(1 for d in draws if d["n"] == n) for n in levels},
+               steps_per_s_alone=total / t_alone, steps_per_s_ensemble=total / t_ens,
+               seconds_alone=t_alone, seconds_ensemble=t_ens,
+               vcycles_per_solve=vc_plain, bitwise_equal_to_alone=bool(same))
+    if proj_slots:
+        engines = [make(i, proj_slots) for i in range(members)]
+        sync()
+        t0 = time.perf_counter()
+        lib.ensemble_step(engines, steps)
+        sync()
+        t_proj = time.perf_counter() - t0
+        out["projection"] = dict(slots=proj_slots, steps_per_s=total / t_proj,
+                                 seconds=t_proj,
+                                 vcycles_per_solve=vcycles_per_solve(engines),
+                                 speedup=t_ens / t_proj)
+        for e in engines:
+            e.close()
+        log(f"ensemble with the projection start ({proj_slots} slots): "
+            f"{total / t_proj:.1f} steps/s ({t_ens / t_proj:.2f}x)")
+    return out
 
 
 def run_mixed_2049(n_events=2, tail_steps=200):
@@ -335,13 +359,15 @@ def run_mixed_2049(n_events=2, tail_steps=200):
     import numpy as np
     from qdeqcycle_fdm_amd import config, lib
 
-    def run(n, mode, tmax=None):
+    def run(n, mode, tmax=None, proj=0):
         levels = {1025: 7, 2049: 8}[n]
         pb = config.make_bp1_problem(n, basin=True, levels=levels,
                                      tol_mode=config.TOL_REL, tol=1e-10)
         me = lib.build_mu_equi(pb.mu)
         extra = {} if tmax is None else {"tmax": tmax}
-        eng = make_engine(pb, me, mode, **extra)
+        eng = make_engine(pb, me, mode, max_vcycles=200, **extra)
+        if proj:
+            eng.projection_enable(proj)
         eng.rk43_setup()
         eng.events_enable(1e-3)
         sync()
@@ -364,7 +390,8 @@ def run_mixed_2049(n_events=2, tail_steps=200):
         st = eng.state()
         ev = eng.events()
         eng.close()
-        log(f"{n}^2 {mode}: {info['it']} steps to t={info['t']:.6g} in {sec:.1f} s")
+        tag = f"{mode} + projection ({proj} slots)" if proj else mode
+        log(f"{n}^2 {tag}: {info['it']} steps to t={info['t']:.6g} in {sec:.1f} s")
         return dict(info=info, seconds=sec, slip=st["slip"][1:-1],
                     V=st["V"][1:-1], events=ev)
 
@@ -372,6 +399,7 @@ def run_mixed_2049(n_events=2, tail_steps=200):
     ir = run(2049, "ir32")
     T = ir["info"]["t"]
     f64 = run(2049, "fast64", tmax=T)
+    f64p = run(2049, "fast64", tmax=T, proj=6)
 
     def catalogue(a, b):
         ea = [e["t_onset"] for e in a["events"]]
@@ -399,8 +427,10 @@ def run_mixed_2049(n_events=2, tail_steps=200):
                     vcycles_per_solve=r["info"]["vcycles"] / max(r["info"]["solves"], 1))
 
     return dict(T=T, runs={"1025_fp64": rate(ref), "2049_ir32": rate(ir),
-                           "2049_fp64": rate(f64)},
+                           "2049_fp64": rate(f64),
+                           "2049_fp64_projection_6_slots": rate(f64p)},
                 ir32_vs_fp64_2049=cmp(ir, f64),
+                fp64_projection_vs_fp64_2049=cmp(f64p, f64),
                 ir32_2049_vs_fp64_1025=catalogue(ir, ref))
 
 
@@ -459,6 +489,8 @@ def main():
                     help="only run the 2049^2 mixed-precision workload and its "
                          "accuracy comparison")
     ap.add_argument("--ensemble-mode", default="ir32", choices=("fast64", "ir32"))
+    ap.add_argument("--ensemble-proj-slots", type=int, default=6,
+                    help="also run the ensemble with the projection start (0: no)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="default tile shapes instead of the autotuned ones")
     ap.add_argument("--no-projection", action="store_true",
@@ -504,7 +536,8 @@ def main():
                           "data": "synthetic", "mixed2049": res}), flush=True)
         return
     if args.ensemble:
-        res = run_ensemble(args.ensemble, args.ensemble_steps, args.ensemble_mode)
+        res = run_ensemble(args.ensemble, args.ensemble_steps, args.ensemble_mode,
+                           args.ensemble_proj_slots)
         log(f"ensemble: {res['steps_per_s_ensemble']:.1f} steps/s together, "
             f"{res['steps_per_s_alone']:.1f} one after the other, bitwise equal: "
             f"{res['bitwise_equal_to_alone']}")
