@@ -266,6 +266,11 @@ int qdeq_projection_enable(qdeq_ctx *ctx, int slots);
  * columns kept. */
 int qdeq_projection_fit(qdeq_ctx *ctx, const double *g_new,
                         const double *g_old, int cnt, double *coef);
+/* Fault injection for tests (no reference counterpart): every later
+ * projection start is U_0 + factor (start - U_0), U_0 the newest stored
+ * solution (1 = off).  A factor such as 1e6 or NaN makes the start worse
+ * than the previous solution, so the solves fall back to it. */
+int qdeq_projection_inject(qdeq_ctx *ctx, double factor);
 /* Starts offered and used (shared with the learned-guess counters), and the
  * last fit's ||residual|| / ||fault data change|| and column count. */
 int qdeq_projection_stats(qdeq_ctx *ctx, long long *offered,

@@ -232,7 +232,7 @@ __global__ void k_proj_apply(const double *u, const double *__restrict__ wr,
                              const double *__restrict__ ring, size_t rstride,
                              const double *__restrict__ rr, ProjSlots sl,
                              const double *__restrict__ coef, int N, int K, int P,
-                             double *cand, int copy_bnd) {
+                             double *cand, int copy_bnd, double inject) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
   if (k >= P || j >= N) return;
@@ -248,7 +248,10 @@ __global__ void k_proj_apply(const double *u, const double *__restrict__ wr,
   double acc = u0;
   for (int i = 1; i < sl.cnt; i++)
     acc = acc + coef[i] * (ring[(size_t)sl.s[i] * rstride + c] - u0);
-  cand[c] = acc + w * wr[c];
+  const double v = acc + w * wr[c];
+  // fault injection for tests (qdeq_projection_inject): the start's change
+  // from the newest solution scaled
+  cand[c] = inject == 1.0 ? v : u0 + inject * (v - u0);
 }
 
 // Stores the solution just found, its fault column and its remote value.
@@ -334,7 +337,8 @@ int proj_before_solve(qdeq_ctx *c) {
   // start's interior is the newest stored solution); on the host path into r
   hipLaunchKernelGGL(k_proj_apply, grd, blk, 0, c->stream, c->d[0].u, c->proj_wr,
                      c->proj_u, (size_t)d.N * d.P, c->proj_r, sl, c->proj_coef,
-                     d.N, d.K, d.P, dev ? c->d[0].u : c->d[0].r, dev ? 0 : 1);
+                     d.N, d.K, d.P, dev ? c->d[0].u : c->d[0].r, dev ? 0 : 1,
+                     c->proj_inject);
   if (!dev) launch_resnorm_of(c, c->d[0].r, 2 * 1024, SC_NORM_PRED);
   QDEQ_HIP(c, hipGetLastError());
   c->cand_pending = true;
@@ -437,6 +441,12 @@ int qdeq_projection_fit(qdeq_ctx *c, const double *g_new, const double *g_old,
   for (int i = 0; i < cnt; i++) coef[i] = h[i];
   coef[cnt] = h[PROJ_MAX_SLOTS];
   coef[cnt + 1] = h[PROJ_MAX_SLOTS + 1];
+  return QDEQ_OK;
+}
+
+int qdeq_projection_inject(qdeq_ctx *c, double factor) {
+  if (!c) return QDEQ_ERR_ARG;
+  c->proj_inject = factor;
   return QDEQ_OK;
 }
 

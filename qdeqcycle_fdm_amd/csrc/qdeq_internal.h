@@ -188,6 +188,7 @@ struct qdeq_ctx {
   // each), their fault columns and remote values, the fit's coefficients,
   // and W_R
   int proj_slots = 0, proj_count = 0, proj_head = 0;
+  double proj_inject = 1.0;  // test hook: scales the start's change
   double *proj_u = nullptr, *proj_g = nullptr, *proj_r = nullptr,
          *proj_coef = nullptr, *proj_wr = nullptr;
 };

@@ -43,7 +43,7 @@ EXPORTS = (
     "qdeq_kernel_breakdown", "qdeq_set_tile_shapes", "qdeq_ensemble_step",
     "qdeq_output_open", "qdeq_output_close", "qdeq_events_enable",
     "qdeq_events_get", "qdeq_projection_enable", "qdeq_projection_stats",
-    "qdeq_projection_fit",
+    "qdeq_projection_fit", "qdeq_projection_inject",
     # include/qdeq_guess.h
     "qdeq_guess_default_config", "qdeq_train_default_config",
     "qdeq_guess_create", "qdeq_guess_destroy", "qdeq_guess_last_error",
@@ -209,6 +209,7 @@ def load(path: str = LIB_PATH):
     L.qdeq_projection_enable.argtypes = [_vp, ctypes.c_int]
     L.qdeq_projection_stats.argtypes = [_vp, _llp, _llp, _dp, _ip]
     L.qdeq_projection_fit.argtypes = [_vp, _dp, _dp, ctypes.c_int, _dp]
+    L.qdeq_projection_inject.argtypes = [_vp, ctypes.c_double]
     for name in EXPORTS:
         getattr(L, name)  # every declared symbol must resolve
     _lib = L
@@ -646,4 +647,6 @@ Engine.guess_inject = _engine_guess_inject
 Engine.projection_enable = _engine_projection_enable
 Engine.projection_stats = _engine_projection_stats
 Engine.projection_fit = _engine_projection_fit
+Engine.projection_inject = lambda self, factor: self._check(
+    load().qdeq_projection_inject(self.h, factor))
 Engine.guess_infer = _engine_guess_infer

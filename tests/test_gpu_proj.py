@@ -10,6 +10,9 @@
   time) with fewer than half the V-cycles per solve; nearly every offered
   start is used.
 * It cannot be combined with an attached learned guess.
+* The start decided on the device equals the host decision bit for bit,
+  and an absurd or non-finite start (fault injection) is never used: the
+  run then equals the plain run bit for bit on both paths.
 """
 import numpy as np
 import pytest
@@ -136,3 +139,28 @@ def test_device_start_equals_host_start(slots, monkeypatch):
     assert pa == pb_
     for k in ("slip", "psi", "V", "U"):
         np.testing.assert_array_equal(a[k], b[k])
+
+
+@pytest.mark.parametrize("factor", [1e6, float("nan")])
+@pytest.mark.parametrize("host", [False, True])
+def test_bad_projection_start_falls_back_bit_for_bit(factor, host, monkeypatch):
+    """A start made absurd or non-finite (fault injection) is never used:
+    each solve falls back to the previous solution (restored over u's
+    interior on the device path), and the run equals the plain run bit for
+    bit."""
+    if host:
+        monkeypatch.setenv("QDEQ_HOST_START", "1")
+    else:
+        monkeypatch.delenv("QDEQ_HOST_START", raising=False)
+    with _engine(129) as a, _engine(129) as b:
+        b.projection_enable(6)
+        b.projection_inject(factor)
+        for e in (a, b):
+            e.rk43_setup()
+            e.rk43_step(40)
+        sa, sb = a.state(), b.state()
+        stats = b.projection_stats()
+        assert stats["offered"] > 0 and stats["used"] == 0
+        assert sa["info"]["vcycles"] == sb["info"]["vcycles"]
+        for k in ("slip", "psi", "V", "U"):
+            np.testing.assert_array_equal(sa[k], sb[k])
