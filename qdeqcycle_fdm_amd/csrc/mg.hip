@@ -176,16 +176,28 @@ __global__ __launch_bounds__(NORM_THREADS) void k_resnorm_partial(
   if (gate && gate[0]) return;  // batched solve already converged
   __shared__ double sh[NORM_THREADS];
   double acc = 0.0;
+  constexpr int UNR = 4;  // as in k_start_partials
   for (int j = blockIdx.x; j < N; j += gridDim.x) {
     const int jS = (j == 0) ? 1 : j - 1;
     const int jN = (j == N - 1) ? N - 2 : j + 1;
-    for (int k = 1 + threadIdx.x; k <= K - 2; k += NORM_THREADS) {
-      const double r = residual_at(u, cx, cz, j, jS, jN, k, P, DH2);
-      acc += r * r;
-      if (STORE32) {
-        const size_t c = (size_t)j * P + k;
-        r32[c] = (float)r;
-        e32[c] = 0.0f;
+    for (int k0 = 1 + threadIdx.x; k0 <= K - 2; k0 += UNR * NORM_THREADS) {
+      double r[UNR];
+#pragma unroll
+      for (int q = 0; q < UNR; q++) {
+        const int k = k0 + q * NORM_THREADS;
+        r[q] = k <= K - 2 ? residual_at(u, cx, cz, j, jS, jN, k, P, DH2) : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < UNR; q++) {
+        const int k = k0 + q * NORM_THREADS;
+        if (k <= K - 2) {
+          acc += r[q] * r[q];
+          if (STORE32) {
+            const size_t c = (size_t)j * P + k;
+            r32[c] = (float)r[q];
+            e32[c] = 0.0f;
+          }
+        }
       }
     }
   }
@@ -293,29 +305,35 @@ __global__ __launch_bounds__(NORM_THREADS) void k_start_partials(
     const double *__restrict__ cand, const double *__restrict__ cx,
     const double *__restrict__ cz, int N, int K, int P, double DH2, int bd,
     double *__restrict__ partial) {
-  __shared__ double sh[NORM_THREADS];
-  double acc = 0.0, accc = 0.0;
+  __shared__ double sh[3][NORM_THREADS];
+  double acc = 0.0, accc = 0.0, accb = 0.0;
+  // each thread's points of a row are evaluated together (their loads in
+  // flight at once), then added in order
+  constexpr int UNR = 4;
   for (int j = blockIdx.x; j < N; j += gridDim.x) {
     const int jS = (j == 0) ? 1 : j - 1;
     const int jN = (j == N - 1) ? N - 2 : j + 1;
-    for (int k = 1 + threadIdx.x; k <= K - 2; k += NORM_THREADS) {
-      const double r = residual_at2(ui, u, cx, cz, j, jS, jN, k, K, P, DH2);
-      acc += r * r;
-      if (cand) {
-        const double rc = residual_at(cand, cx, cz, j, jS, jN, k, P, DH2);
-        accc += rc * rc;
+    for (int k0 = 1 + threadIdx.x; k0 <= K - 2; k0 += UNR * NORM_THREADS) {
+      double r[UNR], rc[UNR];
+#pragma unroll
+      for (int q = 0; q < UNR; q++) {
+        const int k = k0 + q * NORM_THREADS;
+        r[q] = rc[q] = 0.0;
+        if (k <= K - 2) {
+          r[q] = residual_at2(ui, u, cx, cz, j, jS, jN, k, K, P, DH2);
+          if (cand) rc[q] = residual_at(cand, cx, cz, j, jS, jN, k, P, DH2);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < UNR; q++) {
+        if (k0 + q * NORM_THREADS <= K - 2) {
+          acc += r[q] * r[q];
+          if (cand) accc += rc[q] * rc[q];
+        }
       }
     }
   }
-  double s = block_sum_256(acc, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
-  if (cand) {
-    __syncthreads();
-    s = block_sum_256(accc, sh);
-    if (threadIdx.x == 0) partial[2 * NORM_BLOCKS + blockIdx.x] = s;
-  }
   if (bd && blockIdx.x < BDRES_BLOCKS) {
-    double accb = 0.0;
     for (int j = blockIdx.x * NORM_THREADS + threadIdx.x; j < N;
          j += BDRES_BLOCKS * NORM_THREADS) {
       const size_t row = (size_t)j * P;
@@ -329,9 +347,25 @@ __global__ __launch_bounds__(NORM_THREADS) void k_start_partials(
         accb += r * r;
       }
     }
+  }
+  // the three block sums in one tree (each the order of block_sum_256)
+  const int tid = threadIdx.x;
+  sh[0][tid] = acc;
+  sh[1][tid] = accc;
+  sh[2][tid] = accb;
+  __syncthreads();
+  for (int st = NORM_THREADS / 2; st > 0; st >>= 1) {
+    if (tid < st) {
+      sh[0][tid] += sh[0][tid + st];
+      sh[1][tid] += sh[1][tid + st];
+      sh[2][tid] += sh[2][tid + st];
+    }
     __syncthreads();
-    s = block_sum_256(accb, sh);
-    if (threadIdx.x == 0) partial[NORM_BLOCKS + blockIdx.x] = s;
+  }
+  if (tid == 0) {
+    partial[blockIdx.x] = sh[0][0];
+    if (cand) partial[2 * NORM_BLOCKS + blockIdx.x] = sh[1][0];
+    if (bd && blockIdx.x < BDRES_BLOCKS) partial[NORM_BLOCKS + blockIdx.x] = sh[2][0];
   }
 }
 
@@ -345,22 +379,28 @@ __global__ __launch_bounds__(NORM_THREADS) void k_start_partials(
 __global__ __launch_bounds__(NORM_THREADS) void k_start_gate(
     const double *__restrict__ partial, int cand, int rel, double tol, int cap,
     double *__restrict__ scal, double *__restrict__ mb, int *__restrict__ gate) {
-  __shared__ double sh[NORM_THREADS];
-  double a = 0.0;
-  for (int i = threadIdx.x; i < NORM_BLOCKS; i += NORM_THREADS) a += partial[i];
-  const double su = block_sum_256(a, sh);
-  __syncthreads();
-  double b = 0.0;
+  // the three sums of k_norm_final's order, in one tree
+  __shared__ double sh[3][NORM_THREADS];
+  const int tid = threadIdx.x;
+  double a = 0.0, b = 0.0, cc = 0.0;
+  for (int i = tid; i < NORM_BLOCKS; i += NORM_THREADS) a += partial[i];
   if (rel)
-    for (int i = threadIdx.x; i < BDRES_BLOCKS; i += NORM_THREADS)
-      b += partial[NORM_BLOCKS + i];
-  const double sb = block_sum_256(b, sh);
-  __syncthreads();
-  double cc = 0.0;
+    for (int i = tid; i < BDRES_BLOCKS; i += NORM_THREADS) b += partial[NORM_BLOCKS + i];
   if (cand)
-    for (int i = threadIdx.x; i < NORM_BLOCKS; i += NORM_THREADS)
-      cc += partial[2 * NORM_BLOCKS + i];
-  const double sc = block_sum_256(cc, sh);
+    for (int i = tid; i < NORM_BLOCKS; i += NORM_THREADS) cc += partial[2 * NORM_BLOCKS + i];
+  sh[0][tid] = a;
+  sh[1][tid] = b;
+  sh[2][tid] = cc;
+  __syncthreads();
+  for (int st = NORM_THREADS / 2; st > 0; st >>= 1) {
+    if (tid < st) {
+      sh[0][tid] += sh[0][tid + st];
+      sh[1][tid] += sh[1][tid + st];
+      sh[2][tid] += sh[2][tid + st];
+    }
+    __syncthreads();
+  }
+  const double su = sh[0][0], sb = sh[1][0], sc = sh[2][0];
   if (threadIdx.x == 0) {
     const double du = sqrt(su), dc = sqrt(sc);
     const double r0 = rel ? sqrt(sb) : du;
