@@ -278,7 +278,7 @@ class Sim:
 
     def __init__(self, problem, mu_equi_arr=None, *, tol_mode=None, tol=None,
                  max_vcycles=None, n_cycles=None, cycle=None, n_pre=None,
-                 n_post=None, n_coarse=None):
+                 n_post=None, n_coarse=None, relax=None):
         p = problem.params
         self.problem = problem
         me = mu_equi(problem.mu) if mu_equi_arr is None else mu_equi_arr
@@ -287,7 +287,7 @@ class Sim:
         self.params = OrcParams(
             ni=p.ni, nk=p.nk, dh=p.dh, n_levels=p.n_levels,
             n_cycles=p.n_cycles if n_cycles is None else n_cycles,
-            relax=p.relax,
+            relax=p.relax if relax is None else relax,
             tol_mode=p.tol_mode if tol_mode is None else tol_mode,
             tol=p.tol if tol is None else tol,
             max_vcycles=p.max_vcycles if max_vcycles is None else max_vcycles,

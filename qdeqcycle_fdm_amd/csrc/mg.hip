@@ -793,6 +793,7 @@ int mg_begin(qdeq_ctx *c) {
   c->cand_pending = false;
   m.it = 0;
   m.sample = false;
+  m.launched = 0;
   m.dev_start = device_start(c);
   m.start_read = false;
   if (m.dev_start) {
@@ -898,6 +899,7 @@ static int launch_batch(qdeq_ctx *c, int k) {
     // events recorded around the timed kernel
     const bool sample = c->timing != 0 && c->timing != QDEQ_TIME_FAULT &&
                         use_tiled(c) && (c->timed_seq++ % kTimingEvery) == 0;
+    m.launched++;
     if (sample) {
       c->emit_events = true;
       c->gate_ptr = c->gate;
@@ -906,6 +908,7 @@ static int launch_batch(qdeq_ctx *c, int k) {
       c->emit_events = false;
       QDEQ_HIP(c, hipGetLastError());
       m.sample = true;
+      m.sample_ordinal = m.launched;
     } else if (c->direct_launch) {
       c->gate_ptr = c->gate;
       launch_iteration(c);
@@ -979,9 +982,12 @@ int mg_poll(qdeq_ctx *c, bool block, int *done) {
     if (m.phase != 2) return set_error(c, QDEQ_ERR_STATE, "multigrid: no solve in progress");
     if ((st = seq_check(c, m.seq_target, block, &ready)) || !ready) goto out;
     if (m.sample) {
+      // counted only if the timed V-cycle ran (the solve had not stopped
+      // before it)
       float ms = 0.0f;
       QDEQ_HIP(c, hipEventSynchronize(c->ev_t1));
-      if (hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1) == hipSuccess) {
+      if ((int)c->mbox_host[MB_COUNT] >= m.sample_ordinal &&
+          hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1) == hipSuccess) {
         c->timed_ms += ms;
         c->timed_calls += 1;
       }

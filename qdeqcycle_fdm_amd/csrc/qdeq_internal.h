@@ -75,6 +75,9 @@ struct MgSolve {
   bool ir = false, cand = false, sample = false;
   // start decided on the device (k_start_gate); its norms not yet read back
   bool dev_start = false, start_read = false;
+  // V-cycles launched in this solve, and the ordinal of the timed one (a
+  // timed cycle the gate skipped is not counted)
+  int launched = 0, sample_ordinal = 0;
   int it = 0;
   double delta = 0, delta0 = 0, r0 = 0, thr = 0;
   long long seq_target = 0;  // sequence number ending the launched batch

@@ -25,9 +25,12 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
+# the fast cycle of the bench: V(1,3), relaxation 1.15
 MODES = {
-    "fast64": dict(cycle=1, precision=0, n_pre=2, n_post=2, n_coarse=0, tiles=(64, 16)),
-    "ir32": dict(cycle=1, precision=1, n_pre=2, n_post=2, n_coarse=0, tiles=(64, 32)),
+    "fast64": dict(cycle=1, precision=0, n_pre=1, n_post=3, n_coarse=0, relax=1.15,
+                   tiles=(64, 16)),
+    "ir32": dict(cycle=1, precision=1, n_pre=1, n_post=3, n_coarse=0, relax=1.15,
+                 tiles=(64, 32)),
 }
 LEVELS = {257: 5, 513: 6, 1025: 7, 2049: 8}
 
@@ -48,7 +51,14 @@ def main(argv=None):
     ap.add_argument("--rk-warmup", type=int, default=100)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--proj-slots", type=int, default=0)
+    ap.add_argument("--n-pre", type=int, default=None,
+                    help="pre-smoothing sweeps (default: the mode's)")
+    ap.add_argument("--n-post", type=int, default=None)
+    ap.add_argument("--relax", type=float, default=None)
     args = ap.parse_args(argv)
+    for key in ("n_pre", "n_post", "relax"):
+        if getattr(args, key) is not None:
+            MODES[args.mode][key] = getattr(args, key)
 
     from qdeqcycle_fdm_amd import config, lib
     pb = config.make_bp1_problem(args.n, basin=True, levels=LEVELS[args.n],

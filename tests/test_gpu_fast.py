@@ -6,6 +6,8 @@
   residual and the FP64 oracle's solution to the accuracy that residual
   implies; tiled and untiled IR are bit-identical.
 * The RK43 driver in the fast FP64 mode matches the oracle step for step.
+* The same for the production cycle of the bench, V(1,3) with relaxation
+  1.15.
 """
 import numpy as np
 import pytest
@@ -237,3 +239,54 @@ def test_ir_and_fp64_at_2049():
             out[prec] = V
     scale = np.max(np.abs(out[config.PREC_FP64]))
     assert np.max(np.abs(out[config.PREC_FP32_IR] - out[config.PREC_FP64])) <= 1e-6 * scale
+
+
+# the bench's production cycle: V(1,3), over-relaxed red/black sweeps
+PROD = dict(cycle=config.CYCLE_FAST, n_pre=1, n_post=3, n_coarse=0, relax=1.15)
+
+
+@pytest.mark.parametrize("n,levels", [(129, 4), (257, 5)])
+def test_production_cycle_equals_oracle(n, levels):
+    """V(1,3) with relaxation 1.15 on the tiled kernels (several tile
+    shapes) equals the oracle's fast cycle with the same shape and factor."""
+    pb = config.make_bp1_problem(n, basin=True, levels=levels)
+    me = oracle.mu_equi(pb.mu)
+    U = _perturbed(pb, 5)
+    p = pb.params
+    mg = oracle.MG(p.ni, p.nk, p.dh, p.n_levels, me, relax=PROD["relax"])
+    mg.set_cycle(PROD["n_pre"], PROD["n_post"], PROD["n_coarse"], True)
+    ref = U.copy()
+    st, vc, _, _ = mg.solve(ref, oracle.ORC_TOL_ABS, 1e-300, 3)
+    assert vc == 3
+    for tile in ((64, 16), (32, 8), (128, 4)):
+        with lib.Engine(pb, me, tol=1e-300, max_vcycles=3, **PROD) as eng:
+            eng.set_tiles(True, *tile)
+            V = U.copy()
+            try:
+                eng.ode_solve(V)
+            except lib.QdeqError as e:
+                assert e.status == lib.QDEQ_ERR_MG_NOT_CONVERGED
+        np.testing.assert_array_equal(V, ref, err_msg=str(tile))
+
+
+def test_rk43_production_cycle_matches_oracle():
+    pb = config.make_bp1_problem(65, basin=True, levels=3, tol_mode=config.TOL_REL,
+                                 tol=1e-10)
+    me = oracle.mu_equi(pb.mu)
+    oracle.set_libm(False)
+    sim = oracle.Sim(pb, me, cycle=1, n_pre=PROD["n_pre"], n_post=PROD["n_post"],
+                     n_coarse=PROD["n_coarse"], relax=PROD["relax"])
+    sim.init()
+    with lib.Engine(pb, me, **PROD) as eng:
+        assert eng.uses_tiled_cycle
+        eng.rk43_setup()
+        for s in range(40):
+            sim.step()
+            info = eng.rk43_step(1)
+            ref = sim.scalars()
+            for key in ("t", "dt", "totErr", "V_max", "attempts", "vcycles"):
+                assert info[key] == ref[key], (s, key, info[key], ref[key])
+        st = eng.state()
+    ar = sim.arrays()
+    for key in ("slip", "psi", "V", "U"):
+        np.testing.assert_array_equal(st[key], ar[key], err_msg=key)
