@@ -1,9 +1,9 @@
 """GPU checks of ensemble stepping (qdeq_ensemble_step): members with uneven
 grids and seeded frictional parameters, stepped together from one host
 thread on their own streams, are each bit-identical to the same simulation
-run alone; a member that fails (forced from the host: a V-cycle cap of one
-at an unreachable tolerance) reports its status without disturbing the
-others."""
+run alone (FP64, FP32-IR, and FP64 with the projection start); a member
+that fails (forced from the host: a V-cycle cap of one at an unreachable
+tolerance) reports its status without disturbing the others."""
 import numpy as np
 import pytest
 
@@ -25,24 +25,27 @@ def _members(count, seed):
     return out
 
 
-def _engine(pb, precision=0, **kw):
+def _engine(pb, precision=0, slots=0, **kw):
     eng = lib.Engine(pb, cycle=config.CYCLE_FAST, n_pre=2, n_post=2, n_coarse=0,
                      precision=precision, **kw)
     eng.set_tiles(True, 64, 16 if precision == 0 else 32)
+    if slots:
+        eng.projection_enable(slots)
     eng.rk43_setup()
     return eng
 
 
-@pytest.mark.parametrize("precision", [config.PREC_FP64, config.PREC_FP32_IR])
-def test_ensemble_members_equal_solo_runs(precision):
+@pytest.mark.parametrize("precision,slots", [(config.PREC_FP64, 0), (config.PREC_FP32_IR, 0),
+                                             (config.PREC_FP64, 6)])
+def test_ensemble_members_equal_solo_runs(precision, slots):
     pbs = _members(6, seed=3)
     steps = 40
     solo = []
     for pb in pbs:
-        with _engine(pb, precision) as e:
+        with _engine(pb, precision, slots) as e:
             info = e.rk43_step(steps)
             solo.append((info, e.state()))
-    engines = [_engine(pb, precision) for pb in pbs]
+    engines = [_engine(pb, precision, slots) for pb in pbs]
     status = lib.ensemble_step(engines, steps)
     assert status == [lib.QDEQ_OK] * len(pbs)
     for (info, st), e in zip(solo, engines):
@@ -53,7 +56,7 @@ def test_ensemble_members_equal_solo_runs(precision):
             np.testing.assert_array_equal(s2[k], st[k])
     # a second call continues from where the first stopped
     status = lib.ensemble_step(engines, 5)
-    with _engine(pbs[0], precision) as e:
+    with _engine(pbs[0], precision, slots) as e:
         e.rk43_step(steps + 5)
         ref = e.state()
     np.testing.assert_array_equal(engines[0].state()["slip"], ref["slip"])
