@@ -30,6 +30,9 @@ constexpr int TX_THREADS = 64;   // one wavefront across a tile row
 constexpr int TY_THREADS = 16;   // 16 rows of lanes: 1024 threads per tile
 constexpr int TILE_THREADS = TX_THREADS * TY_THREADS;
 constexpr int TAIL_THREADS = 1024;
+constexpr int TAIL_MAX_N = 33;  // levels with N, K <= this run in the tail
+// the tail loads a level's arrays in at most two passes of its threads
+static_assert((TAIL_MAX_N + 1) * TAIL_MAX_N <= 2 * TAIL_THREADS, "tail load passes");
 
 template <typename T, bool EXACT>
 __device__ inline T rbgs_update(T m1, T m2, T m3, T m4, T uW, T uE, T uS,
@@ -78,6 +81,73 @@ __device__ inline void tile_load(Tile<T> &t, const T *__restrict__ u,
       if (in) vx = cx[grow + k];
       if (jinz && kin) vz = cz[grow + k];
       if (lr < t.R) {
+        t.su[s] = vu;
+        if (HAS_F) t.sf[s] = vf;
+        t.scx[s] = vx;
+      }
+      t.scz[s] = vz;
+    }
+  }
+}
+
+// tile_load for k_up with the prolongation folded in: su = u + (P e) at
+// every point the bilinear prolongation reaches (prolongate_kernel,
+// odeSolver.cpp:1017-1043; the same single addition per point as a separate
+// pass over su), so the coarse correction's loads are in flight with the
+// tile's and the pass and its barrier go away.
+template <typename T, bool HAS_F, bool FULL_TOP>
+__device__ inline void tile_load_up(Tile<T> &t, const T *__restrict__ u,
+                                    const T *__restrict__ f,
+                                    const T *__restrict__ cx,
+                                    const T *__restrict__ cz, int N, int K, int P,
+                                    const T *__restrict__ e, int Nc, int Kc,
+                                    int Pc) {
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  for (int lr = ty; lr <= t.R; lr += blockDim.y) {
+    const int j = t.j0 + lr;
+    const bool jin = (j >= 0) && (j < N) && lr < t.R;
+    const bool jinz = (j >= 0) && (j <= N);
+    const size_t grow = (size_t)j * P;
+    // the coarse rows of the point's correction (row cj + 1 only for odd j)
+    const int cj = j >> 1, dj = j & 1;
+    const bool jp = jin && (cj < Nc - 1 || (FULL_TOP && cj == Nc - 1 && dj == 0));
+    const size_t r0 = (size_t)(jp ? cj : 0) * Pc;
+    const size_t r1 = (size_t)(jp ? cj + dj : 0) * Pc;
+#pragma unroll
+    for (int cc = 0; cc < 5; cc++) {
+      const int lc = tx + TX_THREADS * cc;
+      if (lc >= t.C) break;
+      const int k = t.k0 + lc;
+      const bool kin = (k >= 0) && (k < K);
+      const int s = lr * t.LS + lc;
+      const bool in = jin && kin;
+      const int ck = k >> 1, dk = k & 1;
+      const bool pr = jp && kin && ck < Kc - 1;
+      T vu = T(0), vf = T(0), vx = T(0), vz = T(0);
+      T e00 = T(0), e01 = T(0), e10 = T(0), e11 = T(0);
+      if (in) vu = u[grow + k];
+      if (HAS_F && in) vf = f[grow + k];
+      if (in) vx = cx[grow + k];
+      if (jinz && kin) vz = cz[grow + k];
+      if (pr) {
+        e00 = e[r0 + ck];
+        e01 = e[r0 + ck + dk];
+        e10 = e[r1 + ck];
+        e11 = e[r1 + ck + dk];
+      }
+      if (lr < t.R) {
+        if (pr) {
+          T add;
+          if (!dj && !dk)
+            add = e00;
+          else if (!dj)
+            add = T(0.5) * (e00 + e01);
+          else if (!dk)
+            add = T(0.5) * (e00 + e10);
+          else
+            add = T(0.25) * (e00 + e01 + e10 + e11);
+          vu += add;
+        }
         t.su[s] = vu;
         if (HAS_F) t.sf[s] = vf;
         t.scx[s] = vx;
@@ -245,35 +315,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_up(
   extern __shared__ __align__(16) unsigned char smem[];
   const int H = 2 * nu + (MODE == UP_NORM ? 1 : 0);
   Tile<T> t = make_tile<T>(smem, TX, TY, H, HAS_F);
-  tile_load<T, HAS_F>(t, uin, f, cx, cz, N, K, P, false);
-  __syncthreads();
-  // u += P e (prolongate_kernel, odeSolver.cpp:1017-1043), point by point
-  for (int lr = threadIdx.y; lr < t.R; lr += blockDim.y) {
-    const int j = t.j0 + lr;
-    if (j < 0 || j >= N) continue;
-    const int cj = j >> 1, dj = j & 1;
-    if (!(cj < Nc - 1 || (FULL_TOP && cj == Nc - 1 && dj == 0))) continue;
-    const T *erow = e + (size_t)cj * Pc;
-    for (int lc = threadIdx.x; lc < t.C; lc += TX_THREADS) {
-      const int k = t.k0 + lc;
-      if (k < 0 || k >= K) continue;
-      const int ck = k >> 1, dk = k & 1;
-      if (ck >= Kc - 1) continue;
-      const T *ec = erow + ck;
-      const T e00 = ec[0];
-      T add;
-      if (!dj && !dk) {
-        add = e00;
-      } else if (!dj) {
-        add = T(0.5) * (e00 + ec[1]);
-      } else if (!dk) {
-        add = T(0.5) * (e00 + ec[Pc]);
-      } else {
-        add = T(0.25) * (e00 + ec[1] + ec[Pc] + ec[Pc + 1]);
-      }
-      t.su[lr * t.LS + lc] += add;
-    }
-  }
+  // u + P e (prolongate_kernel, odeSolver.cpp:1017-1043), loaded together
+  tile_load_up<T, HAS_F, FULL_TOP>(t, uin, f, cx, cz, N, K, P, e, Nc, Kc, Pc);
   __syncthreads();
   tile_sweeps<T, HAS_F, EXACT>(t, nu, N, K, DH2, relax);
   double acc = 0.0;
@@ -325,10 +368,15 @@ struct TailArgs {
   const int *gate;      // skip flag of a batched solve, or null
 };
 
+// compact arrays of a tail level, stride K: u, f, cx and cz (N+1 rows)
 template <typename T>
 struct TailBufs {
-  T *u, *f, *cx, *cz;  // compact, stride K; cz has N+1 rows
+  T *u, *f, *cx, *cz;
 };
+
+__host__ __device__ inline size_t tail_level_words(int N, int K) {
+  return 4 * (size_t)N * K + K;
+}
 
 // LDS arrays of tail level l, recomputed from the level dimensions (uniform
 // values) at each use: a per-level table indexed by a loop variable would
@@ -336,18 +384,16 @@ struct TailBufs {
 template <typename T>
 __device__ inline TailBufs<T> tail_bufs(T *base, const TailArgs &a, int l) {
   size_t off = 0;
-  for (int i = 0; i < l; i++)
-    off += 4 * (size_t)a.lv[i].N * a.lv[i].K + a.lv[i].K;
+  for (int i = 0; i < l; i++) off += tail_level_words(a.lv[i].N, a.lv[i].K);
   const size_t n = (size_t)a.lv[l].N * a.lv[l].K;
-  return TailBufs<T>{base + off, base + off + n, base + off + 2 * n,
-                     base + off + 3 * n};
+  T *b = base + off;
+  return TailBufs<T>{b, b + n, b + 2 * n, b + 3 * n};
 }
 
 template <typename T>
 __device__ inline T *tail_end(T *base, const TailArgs &a) {
   size_t off = 0;
-  for (int i = 0; i < a.nlev; i++)
-    off += 4 * (size_t)a.lv[i].N * a.lv[i].K + a.lv[i].K;
+  for (int i = 0; i < a.nlev; i++) off += tail_level_words(a.lv[i].N, a.lv[i].K);
   return base + off;
 }
 
@@ -391,48 +437,84 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   T *base = reinterpret_cast<T *>(smem);
   T *Rs = tail_end(base, a);  // residual scratch (entry level size)
-  T *Ps = Rs + (a.lv[0].N * a.lv[0].K > a.cn ? a.lv[0].N * a.lv[0].K : a.cn);
-  // Ps: 1024 partial sums of the exact coarsest solve
+  T *Gs = Rs + (a.lv[0].N * a.lv[0].K > a.cn ? a.lv[0].N * a.lv[0].K : a.cn);
+  T *Ps = Gs + a.cn;
+  // Gs: right-hand side DH^2 f of the exact coarsest solve; Ps: its 1024
+  // partial sums
   const int tid = threadIdx.x;
-  // coefficients of every tail level, and the entry level's right-hand side
+  const T relax = (T)a.relax;
+  // Global loads through address-space-1 pointers, so the compiler may keep
+  // them in flight across the LDS stores.  The entry level's right-hand side
+  // is loaded first, together with the first level's coefficients.
+  typedef const __attribute__((address_space(1))) T GT;
+  const TailLevel &L0 = a.lv[0];
+  T fv[2];
+  bool fin[2];
+#pragma unroll
+  for (int it = 0; it < 2; it++) {
+    // clamped indices: every load is issued, unused values are dropped at
+    // the LDS store
+    const int n0 = L0.N * L0.K;
+    const int idx = tid + it * TAIL_THREADS;
+    const int ii = idx < n0 ? idx : n0 - 1;
+    const int j = ii / L0.K, k = ii - j * L0.K;
+    const int kc = k < 1 ? 1 : (k > L0.K - 2 ? L0.K - 2 : k);
+    fv[it] = ((GT *)a.f_in)[(size_t)j * L0.P + kc];
+    fin[it] = k == kc;
+  }
   for (int l = 0; l < a.nlev; l++) {
     const TailLevel &L = a.lv[l];
     const TailBufs<T> b = tail_bufs(base, a, l);
-    const T *cx = reinterpret_cast<const T *>(L.cx);
-    const T *cz = reinterpret_cast<const T *>(L.cz);
-    for (int idx = tid; idx < (L.N + 1) * L.K; idx += blockDim.x) {
-      const int j = idx / L.K, k = idx - j * L.K;
-      b.cz[idx] = cz[(size_t)j * L.P + k];
-      if (j < L.N) b.cx[idx] = cx[(size_t)j * L.P + k];
+    GT *cx = (GT *)L.cx;
+    GT *cz = (GT *)L.cz;
+    const int nz = (L.N + 1) * L.K, nx = L.N * L.K;
+    T vx[2], vz[2];
+#pragma unroll
+    for (int it = 0; it < 2; it++) {
+      const int idx = tid + it * TAIL_THREADS;
+      const int ii = idx < nz ? idx : nz - 1;
+      const int j = ii / L.K, k = ii - j * L.K;
+      const int jx = j < L.N ? j : L.N - 1;
+      vz[it] = cz[(size_t)j * L.P + k];
+      vx[it] = cx[(size_t)jx * L.P + k];
+    }
+#pragma unroll
+    for (int it = 0; it < 2; it++) {
+      const int idx = tid + it * TAIL_THREADS;
+      if (idx < nz) b.cz[idx] = vz[it];
+      if (idx < nx) b.cx[idx] = vx[it];
     }
   }
   {
-    const TailLevel &L0 = a.lv[0];
     const TailBufs<T> b = tail_bufs(base, a, 0);
-    const T *fin = reinterpret_cast<const T *>(a.f_in);
-    for (int idx = tid; idx < L0.N * L0.K; idx += blockDim.x) {
-      const int j = idx / L0.K, k = idx - j * L0.K;
-      b.u[idx] = T(0);
-      b.f[idx] = (k >= 1 && k <= L0.K - 2) ? fin[(size_t)j * L0.P + k] : T(0);
+#pragma unroll
+    for (int it = 0; it < 2; it++) {
+      const int idx = tid + it * TAIL_THREADS;
+      if (idx < L0.N * L0.K) {
+        b.u[idx] = T(0);
+        b.f[idx] = fin[it] ? fv[it] : T(0);
+      }
     }
   }
   __syncthreads();
-  const T relax = (T)a.relax;
   for (int l = 0; l < a.nlev; l++) {
     const TailLevel &L = a.lv[l];
     const T DH2 = (T)L.DH2;
     const TailBufs<T> Bl = tail_bufs(base, a, l);
     if (l == a.nlev - 1) {
       if (a.minv) {
-        // e = B^{-1} (DH^2 f), rows summed in column order
+        // e = B^{-1} (DH^2 f), rows summed in column order; Gs was filled by
+        // the restriction onto this level unless it is the entry level
         const T *minv = reinterpret_cast<const T *>(a.minv);
         const int W = L.K - 2;
-        for (int i = tid; i < a.cn; i += blockDim.x) {
-          const int j = i / W, k = i - j * W + 1;
-          Rs[i] = DH2 * Bl.f[j * L.K + k];
+        if (l == 0) {
+          for (int i = tid; i < a.cn; i += blockDim.x) {
+            const int j = i / W, k = i - j * W + 1;
+            Gs[i] = DH2 * Bl.f[j * L.K + k];
+          }
+          __syncthreads();
         }
-        __syncthreads();
-        coarse_matvec<T>(minv, Rs, Ps, a.cn, Bl.u, W, L.K, (size_t)L.K);
+        coarse_matvec<T>(minv, Gs, Ps, a.cn, Bl.u, W, L.K, (size_t)L.K);
       } else {
         const bool one_wave = L.N * L.K <= 512;
         tail_smooth<T, EXACT>(Bl, L.N, L.K, DH2, relax, a.nu_coarse, one_wave);
@@ -459,6 +541,9 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailArgs a) {
     __syncthreads();
     const TailLevel &C = a.lv[l + 1];
     const TailBufs<T> Bc = tail_bufs(base, a, l + 1);
+    // onto the coarsest level of an exact solve: also its right-hand side
+    const bool direct = a.minv && l + 1 == a.nlev - 1;
+    const T DH2c = (T)C.DH2;
     for (int idx = tid; idx < C.N * C.K; idx += blockDim.x) {
       const int j = idx / C.K, k = idx - j * C.K;
       T v = T(0);
@@ -473,6 +558,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailArgs a) {
             (T(4.0) * rc[kf] + T(2.0) * rc[kf - 1] + T(2.0) * rc[kf + 1] +
              T(2.0) * rs[kf] + T(2.0) * rn[kf] + rs[kf - 1] + rs[kf + 1] +
              rn[kf - 1] + rn[kf + 1]);
+        if (direct) Gs[j * (C.K - 2) + (k - 1)] = DH2c * v;
       }
       Bc.f[idx] = v;
       Bc.u[idx] = T(0);
@@ -516,8 +602,6 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailArgs a) {
 // Host side
 // ---------------------------------------------------------------------------
 
-constexpr int TAIL_MAX_N = 33;
-
 static size_t tile_lds_bytes(int TX, int TY, int H, bool has_f, size_t esz,
                              bool down) {
   const size_t R = TY + 2 * H, C = TX + 2 * H;
@@ -539,11 +623,11 @@ int tail_entry_level(const qdeq_ctx *c) {
 template <typename T>
 static size_t tail_lds_bytes(const qdeq_ctx *c, int lt) {
   size_t n = 0;
-  for (int l = lt; l < c->L; l++)
-    n += 4 * (size_t)c->lv[l].N * c->lv[l].K + c->lv[l].K;
+  for (int l = lt; l < c->L; l++) n += tail_level_words(c->lv[l].N, c->lv[l].K);
   const size_t nr = (size_t)c->lv[lt].N * c->lv[lt].K;
   n += nr > (size_t)c->coarse_n ? nr : (size_t)c->coarse_n;
-  n += 1024;  // partial sums of the exact coarsest solve
+  n += c->coarse_n;  // right-hand side of the exact coarsest solve
+  n += 1024;         // its partial sums
   return n * sizeof(T);
 }
 
@@ -711,6 +795,20 @@ static void launch_one(qdeq_ctx *c, LevelBufs<T> *B, int l, bool down,
 static const int kTuneTX[] = {32, 64, 96, 128, 192, 256};
 static const int kTuneTY[] = {2, 4, 8, 16, 32, 64};
 static const int kTuneTYT[] = {4, 8, 16};
+// Tile sizes that split a level into a whole number of tiles: the grids are
+// 2^k + 1 points wide, so the power-of-two sizes leave a last tile of one
+// or two points.  With one tile per CU per round, a grid of 16 x 16 tiles
+// of 66 points at 1025^2 runs in one round where 96 x 32 needs two.
+static const int kFitTX[] = {4, 8, 12, 16};
+static const int kFitTY[] = {8, 12, 16, 24, 32};
+
+static void add_fitted(std::vector<int> &v, int n, const int *parts, int np) {
+  for (int i = 0; i < np; i++) {
+    int t = (n + parts[i] - 1) / parts[i];
+    t += t & 1;
+    if (t >= 2 && std::find(v.begin(), v.end(), t) == v.end()) v.push_back(t);
+  }
+}
 constexpr int kPartialCap = 16384;  // c->partial entries (level-0 norm blocks)
 
 template <typename T, bool EXACT>
@@ -728,8 +826,12 @@ static int autotune_t(qdeq_ctx *c, LevelBufs<T> *B, bool level0_has_f,
       // every candidate's time, in a fixed order; the first one within 3%
       // of the fastest wins (stable against timing noise between runs)
       std::vector<std::pair<TileShape, float>> times;
-      for (int tx : kTuneTX)
-        for (int ty : kTuneTY)
+      std::vector<int> txs(std::begin(kTuneTX), std::end(kTuneTX));
+      std::vector<int> tys(std::begin(kTuneTY), std::end(kTuneTY));
+      add_fitted(txs, c->lv[l].K, kFitTX, (int)(sizeof(kFitTX) / sizeof(int)));
+      add_fitted(tys, c->lv[l].N, kFitTY, (int)(sizeof(kFitTY) / sizeof(int)));
+      for (int tx : txs)
+        for (int ty : tys)
           for (int tyt : kTuneTYT) {
             const TileShape cand{tx, ty, tyt};
             if (!shape_fits(c, cand, down, hf, sizeof(T))) continue;

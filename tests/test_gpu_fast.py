@@ -197,6 +197,27 @@ def test_pinned_tile_shapes_round_trip_and_validation():
     np.testing.assert_array_equal(V, V2)
 
 
+def test_grid_fitted_tile_shapes_give_identical_solutions():
+    """Tile sizes that split a 2^k + 1 grid into whole tiles (66 = 264/4 at
+    257^2, one tile of 66 x 66 per CU) are among the autotuner's candidates;
+    with them pinned, the solve is bit-identical to the default shapes'."""
+    pb = config.make_bp1_problem(257, basin=True, levels=5,
+                                 tol_mode=config.TOL_REL, tol=1e-10)
+    me = oracle.mu_equi(pb.mu)
+    U = _perturbed(pb, 6)
+    shapes = [((66, 18, 16), (66, 66, 16)), ((34, 10, 8), (44, 22, 16)),
+              ((22, 12, 8), (18, 10, 4))]
+    with lib.Engine(pb, me, **FAST) as eng:
+        eng.set_tiles(True, 64, 16)
+        eng.set_tile_shapes(shapes)
+        assert eng.tile_shapes() == shapes
+        V = U.copy()
+        vc = eng.ode_solve(V)[0]
+    V2, out2 = _gpu_solve(pb, me, U, True)
+    np.testing.assert_array_equal(V, V2)
+    assert vc == out2[0]
+
+
 def test_kernel_timing_does_not_change_results():
     """Timed V-cycles are launched directly instead of from the graph: the
     run must be bit-identical, and the timed kernel must have been timed."""
