@@ -717,6 +717,7 @@ static int rk_advance(qdeq_ctx *c, bool block, int *done) {
         r.staged = false;
         if ((st = guess_before_solve(c))) return st;
         if ((st = proj_before_solve(c))) return st;
+        c->solve_key = (r.final_stage ? 8 : 0) + r.s;
         if ((st = mg_begin(c))) return st;
         r.phase = RK_SOLVING;
         break;

@@ -827,7 +827,10 @@ int mg_begin(qdeq_ctx *c) {
     }
     c->seq_issued += 1;
     QDEQ_HIP(c, hipGetLastError());
-    int st = launch_batch(c, std::min(std::max(c->last_it, 1), 6));
+    const int key = c->solve_key;
+    const int pred = (key >= 0 && c->last_it_key[key] >= 0) ? c->last_it_key[key]
+                                                             : c->last_it;
+    int st = launch_batch(c, std::min(std::max(pred, 1), 6));
     if (st != QDEQ_OK) return st;
     m.phase = 2;
     return QDEQ_OK;
@@ -845,6 +848,8 @@ static int mg_finish(qdeq_ctx *c, int st) {
   MgSolve &m = c->mg;
   c->vcycles += m.it;
   c->last_it = m.it;
+  if (c->solve_key >= 0) c->last_it_key[c->solve_key] = m.it;
+  c->solve_key = -1;
   if (m.ir) c->ir_sweeps += m.it;
   c->solves += 1;
   // contraction per V-cycle seen by this solve (for the next batch sizes)

@@ -119,6 +119,14 @@ struct qdeq_ctx {
   bool direct_launch = false;  // launch V-cycles kernel by kernel (profiling)
   bool host_start = false;     // start decisions on the host (see mg_begin)
   int last_it = 2;             // V-cycles of the last solve (first batch size)
+  // RK43 solves: the stage's key (stage s, +8 for the solve after an
+  // accepted step) while its solve runs, else -1, and the V-cycles of the
+  // last solve of each key (-1: none yet).  The V-cycles a stage needs
+  // repeat from step to step far more closely than they repeat from one
+  // stage to the next, so the first batch of a solve is sized by its key.
+  int solve_key = -1;
+  int last_it_key[16] = {-1, -1, -1, -1, -1, -1, -1, -1,
+                         -1, -1, -1, -1, -1, -1, -1, -1};
   // the pending candidate start is in u's interior; the plain start's
   // interior is then at plain_interior (same layout as u)
   bool cand_in_u = false;
