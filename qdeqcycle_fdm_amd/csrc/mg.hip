@@ -182,10 +182,12 @@ __global__ __launch_bounds__(NORM_THREADS) void k_resnorm_partial(
     const int jN = (j == N - 1) ? N - 2 : j + 1;
     for (int k0 = 1 + threadIdx.x; k0 <= K - 2; k0 += UNR * NORM_THREADS) {
       double r[UNR];
+      // points past the row end evaluate the last interior point instead
+      // (not added), so every load is issued before the first use
 #pragma unroll
       for (int q = 0; q < UNR; q++) {
-        const int k = k0 + q * NORM_THREADS;
-        r[q] = k <= K - 2 ? residual_at(u, cx, cz, j, jS, jN, k, P, DH2) : 0.0;
+        const int k = min(k0 + q * NORM_THREADS, K - 2);
+        r[q] = residual_at(u, cx, cz, j, jS, jN, k, P, DH2);
       }
 #pragma unroll
       for (int q = 0; q < UNR; q++) {
@@ -300,6 +302,7 @@ __global__ __launch_bounds__(NORM_THREADS) void k_norm_gate(
 // partial[2*NORM_BLOCKS..): candidate.
 constexpr int BDRES_BLOCKS = 8;
 
+template <bool CAND>
 __global__ __launch_bounds__(NORM_THREADS) void k_start_partials(
     const double *__restrict__ ui, const double *__restrict__ u,
     const double *__restrict__ cand, const double *__restrict__ cx,
@@ -315,20 +318,19 @@ __global__ __launch_bounds__(NORM_THREADS) void k_start_partials(
     const int jN = (j == N - 1) ? N - 2 : j + 1;
     for (int k0 = 1 + threadIdx.x; k0 <= K - 2; k0 += UNR * NORM_THREADS) {
       double r[UNR], rc[UNR];
+      // points past the row end evaluate the last interior point instead
+      // (not added), so every load is issued before the first use
 #pragma unroll
       for (int q = 0; q < UNR; q++) {
-        const int k = k0 + q * NORM_THREADS;
-        r[q] = rc[q] = 0.0;
-        if (k <= K - 2) {
-          r[q] = residual_at2(ui, u, cx, cz, j, jS, jN, k, K, P, DH2);
-          if (cand) rc[q] = residual_at(cand, cx, cz, j, jS, jN, k, P, DH2);
-        }
+        const int k = min(k0 + q * NORM_THREADS, K - 2);
+        r[q] = residual_at2(ui, u, cx, cz, j, jS, jN, k, K, P, DH2);
+        rc[q] = CAND ? residual_at(cand, cx, cz, j, jS, jN, k, P, DH2) : 0.0;
       }
 #pragma unroll
       for (int q = 0; q < UNR; q++) {
         if (k0 + q * NORM_THREADS <= K - 2) {
           acc += r[q] * r[q];
-          if (cand) accc += rc[q] * rc[q];
+          if (CAND) accc += rc[q] * rc[q];
         }
       }
     }
@@ -364,7 +366,7 @@ __global__ __launch_bounds__(NORM_THREADS) void k_start_partials(
   }
   if (tid == 0) {
     partial[blockIdx.x] = sh[0][0];
-    if (cand) partial[2 * NORM_BLOCKS + blockIdx.x] = sh[1][0];
+    if (CAND) partial[2 * NORM_BLOCKS + blockIdx.x] = sh[1][0];
     if (bd && blockIdx.x < BDRES_BLOCKS) partial[NORM_BLOCKS + blockIdx.x] = sh[2][0];
   }
 }
@@ -808,9 +810,14 @@ int mg_begin(qdeq_ctx *c) {
     c->cand_in_u = false;
     const double *plain = in_u ? c->plain_interior : c->d[0].u;
     const double *cand = !m.cand ? nullptr : (in_u ? c->d[0].u : c->d[0].r);
-    hipLaunchKernelGGL(k_start_partials, dim3(NORM_BLOCKS), dim3(NORM_THREADS), 0,
-                       c->stream, plain, c->d[0].u, cand, c->d[0].cx, c->d[0].cz,
-                       d.N, d.K, d.P, d.DH * d.DH, rel, c->partial);
+    if (cand)
+      hipLaunchKernelGGL(k_start_partials<true>, dim3(NORM_BLOCKS), dim3(NORM_THREADS), 0,
+                         c->stream, plain, c->d[0].u, cand, c->d[0].cx, c->d[0].cz,
+                         d.N, d.K, d.P, d.DH * d.DH, rel, c->partial);
+    else
+      hipLaunchKernelGGL(k_start_partials<false>, dim3(NORM_BLOCKS), dim3(NORM_THREADS), 0,
+                         c->stream, plain, c->d[0].u, cand, c->d[0].cx, c->d[0].cz,
+                         d.N, d.K, d.P, d.DH * d.DH, rel, c->partial);
     hipLaunchKernelGGL(k_start_gate, dim3(1), dim3(NORM_THREADS), 0, c->stream,
                        c->partial, m.cand ? 1 : 0, rel, c->cfg.tol,
                        c->cfg.max_vcycles, c->scal, c->mbox_dev, c->gate);
