@@ -157,31 +157,55 @@ __device__ inline void tile_load_up(Tile<T> &t, const T *__restrict__ u,
   }
 }
 
-// Half-sweep `colour` over the local window [h, R-h) x [h, C-h); lanes run
-// over the points of one colour only.
+// Points (r, c) of a rows x cols window dealt to the workgroup's threads in
+// turn, row by row: thread tid takes points tid, tid + nthr, ...  Every lane
+// has a point even where a row is shorter than a wavefront (a 66-point tile
+// row would leave a third of the lanes idle with one row per wavefront).
+struct FlatWalk {
+  int r, c, dr, dc, cols;
+  __device__ FlatWalk(int cols_) : cols(cols_) {
+    const int nthr = blockDim.x * blockDim.y;
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    r = tid / cols;
+    c = tid - r * cols;
+    dr = nthr / cols;
+    dc = nthr - dr * cols;
+  }
+  __device__ void next() {
+    c += dc;
+    r += dr;
+    if (c >= cols) {
+      c -= cols;
+      r++;
+    }
+  }
+};
+
+// Half-sweep `colour` over the local window [h, R-h) x [h, C-h): the half
+// points of one colour in each row, all rows, dealt out by FlatWalk.  The
+// points of one colour are independent of each other, so the order in
+// which they are updated does not change the result.
 template <typename T, bool HAS_F, bool EXACT>
 __device__ inline void tile_half_sweep(Tile<T> &t, int h, int colour, int N,
                                        int K, T DH2, T relax) {
   const int cols = t.C - 2 * h;
   const int half = (cols + 1) / 2;
-  for (int lr = h + threadIdx.y; lr < t.R - h; lr += blockDim.y) {
+  const int rows = t.R - 2 * h;
+  for (FlatWalk w(half); w.r < rows; w.next()) {
+    const int lr = h + w.r;
     const int j = t.j0 + lr;
-    if (j < 0 || j >= N) continue;
     const int delta = (colour + j + t.k0 + h) & 1;
+    const int lc = h + 2 * w.c + delta;
+    const int k = t.k0 + lc;
+    if (j < 0 || j >= N || lc >= t.C - h || k < 1 || k > K - 2) continue;
     const int lS = (j == 0) ? lr + 1 : lr - 1;
     const int lN = (j == N - 1) ? lr - 1 : lr + 1;
-    for (int q = threadIdx.x; q < half; q += TX_THREADS) {
-      const int lc = h + 2 * q + delta;
-      if (lc >= t.C - h) continue;
-      const int k = t.k0 + lc;
-      if (k < 1 || k > K - 2) continue;
-      const int s = lr * t.LS + lc;
-      const T m1 = t.scx[s], m2 = t.scx[s + 1], m3 = t.scz[s], m4 = t.scz[s + t.LS];
-      const T fv = HAS_F ? DH2 * t.sf[s] : T(0);
-      t.su[s] = rbgs_update<T, EXACT>(m1, m2, m3, m4, t.su[s - 1], t.su[s + 1],
-                                      t.su[lS * t.LS + lc], t.su[lN * t.LS + lc],
-                                      t.su[s], fv, HAS_F, relax);
-    }
+    const int s = lr * t.LS + lc;
+    const T m1 = t.scx[s], m2 = t.scx[s + 1], m3 = t.scz[s], m4 = t.scz[s + t.LS];
+    const T fv = HAS_F ? DH2 * t.sf[s] : T(0);
+    t.su[s] = rbgs_update<T, EXACT>(m1, m2, m3, m4, t.su[s - 1], t.su[s + 1],
+                                    t.su[lS * t.LS + lc], t.su[lN * t.LS + lc],
+                                    t.su[s], fv, HAS_F, relax);
   }
 }
 
@@ -245,46 +269,40 @@ __global__ __launch_bounds__(TILE_THREADS) void k_down(
   // point only) or goes to the scratch block after scz
   T *sr = HAS_F ? t.sf : t.scz + (t.R + 1) * t.LS;
   const int h = H - 1;
-  for (int lr = h + threadIdx.y; lr < t.R - h; lr += blockDim.y) {
+  for (FlatWalk w(t.C - 2 * h); w.r < t.R - 2 * h; w.next()) {
+    const int lr = h + w.r, lc = h + w.c;
     const int j = t.j0 + lr;
-    for (int lc = h + threadIdx.x; lc < t.C - h; lc += TX_THREADS) {
-      const int k = t.k0 + lc;
-      T r = T(0);
-      if (j >= 0 && j < N && k >= 1 && k <= K - 2)
-        r = tile_residual<T, HAS_F>(t, lr, lc, j, N, DH2);
-      sr[lr * t.LS + lc] = r;
-    }
+    const int k = t.k0 + lc;
+    T r = T(0);
+    if (j >= 0 && j < N && k >= 1 && k <= K - 2)
+      r = tile_residual<T, HAS_F>(t, lr, lc, j, N, DH2);
+    sr[lr * t.LS + lc] = r;
   }
   __syncthreads();
   // restriction for the coarse points inside the tile interior
   const int jc0 = (t.j0 + H) / 2, kc0 = (t.k0 + H) / 2;
-  for (int a = threadIdx.y; a < TY / 2; a += blockDim.y) {
-    const int jc = jc0 + a;
-    if (jc >= Nc) continue;
+  for (FlatWalk w(TX / 2); w.r < TY / 2; w.next()) {
+    const int jc = jc0 + w.r;
+    const int kc = kc0 + w.c;
+    if (jc >= Nc || kc < 1 || kc > Kc - 2) continue;
     const int jS = (jc == 0) ? 1 : 2 * jc - 1;
     const int jN = (jc == Nc - 1) ? 2 * jc - 1 : 2 * jc + 1;
     const T *rc = sr + (2 * jc - t.j0) * t.LS - t.k0;
     const T *rs = sr + (jS - t.j0) * t.LS - t.k0;
     const T *rn = sr + (jN - t.j0) * t.LS - t.k0;
-    for (int b = threadIdx.x; b < TX / 2; b += TX_THREADS) {
-      const int kc = kc0 + b;
-      if (kc < 1 || kc > Kc - 2) continue;
-      const int kf = 2 * kc;
-      const T v = T(0.0625) *
-                  (T(4.0) * rc[kf] + T(2.0) * rc[kf - 1] + T(2.0) * rc[kf + 1] +
-                   T(2.0) * rs[kf] + T(2.0) * rn[kf] + rs[kf - 1] + rs[kf + 1] +
-                   rn[kf - 1] + rn[kf + 1]);
-      r2[(size_t)jc * Pc + kc] = v;
-    }
+    const int kf = 2 * kc;
+    const T v = T(0.0625) *
+                (T(4.0) * rc[kf] + T(2.0) * rc[kf - 1] + T(2.0) * rc[kf + 1] +
+                 T(2.0) * rs[kf] + T(2.0) * rn[kf] + rs[kf - 1] + rs[kf + 1] +
+                 rn[kf - 1] + rn[kf + 1]);
+    r2[(size_t)jc * Pc + kc] = v;
   }
   // smoothed interior
-  for (int lr = H + threadIdx.y; lr < H + TY; lr += blockDim.y) {
+  for (FlatWalk w(TX); w.r < TY; w.next()) {
+    const int lr = H + w.r, lc = H + w.c;
     const int j = t.j0 + lr;
-    if (j >= N) continue;
-    for (int lc = H + threadIdx.x; lc < H + TX; lc += TX_THREADS) {
-      const int k = t.k0 + lc;
-      if (k < K) uout[(size_t)j * P + k] = t.su[lr * t.LS + lc];
-    }
+    const int k = t.k0 + lc;
+    if (j < N && k < K) uout[(size_t)j * P + k] = t.su[lr * t.LS + lc];
   }
 }
 
@@ -320,12 +338,11 @@ __global__ __launch_bounds__(TILE_THREADS) void k_up(
   __syncthreads();
   tile_sweeps<T, HAS_F, EXACT>(t, nu, N, K, DH2, relax);
   double acc = 0.0;
-  for (int lr = H + threadIdx.y; lr < H + TY; lr += blockDim.y) {
+  for (FlatWalk w(TX); w.r < TY; w.next()) {
+    const int lr = H + w.r, lc = H + w.c;
     const int j = t.j0 + lr;
-    if (j >= N) continue;
-    for (int lc = H + threadIdx.x; lc < H + TX; lc += TX_THREADS) {
-      const int k = t.k0 + lc;
-      if (k >= K) continue;
+    const int k = t.k0 + lc;
+    if (j < N && k < K) {
       const size_t g = (size_t)j * P + k;
       if (MODE == UP_NORM && k >= 1 && k <= K - 2) {
         const double r = (double)tile_residual<T, HAS_F>(t, lr, lc, j, N, DH2);
