@@ -53,110 +53,6 @@ struct Tile {
   int R, C, LS, j0, k0;
 };
 
-// Loads the tile; columns are visited in unrolled groups of 64 lanes so the
-// global loads of a row are all in flight before the LDS stores.
-template <typename T, bool HAS_F>
-__device__ inline void tile_load(Tile<T> &t, const T *__restrict__ u,
-                                 const T *__restrict__ f,
-                                 const T *__restrict__ cx,
-                                 const T *__restrict__ cz, int N, int K, int P,
-                                 bool zero_u) {
-  const int tx = threadIdx.x, ty = threadIdx.y;
-  for (int lr = ty; lr <= t.R; lr += blockDim.y) {
-    const int j = t.j0 + lr;
-    const bool jin = (j >= 0) && (j < N) && lr < t.R;
-    const bool jinz = (j >= 0) && (j <= N);
-    const size_t grow = (size_t)j * P;
-#pragma unroll
-    for (int cc = 0; cc < 5; cc++) {
-      const int lc = tx + TX_THREADS * cc;
-      if (lc >= t.C) break;
-      const int k = t.k0 + lc;
-      const bool kin = (k >= 0) && (k < K);
-      const int s = lr * t.LS + lc;
-      const bool in = jin && kin;
-      T vu = T(0), vf = T(0), vx = T(0), vz = T(0);
-      if (in && !zero_u) vu = u[grow + k];
-      if (HAS_F && in) vf = f[grow + k];
-      if (in) vx = cx[grow + k];
-      if (jinz && kin) vz = cz[grow + k];
-      if (lr < t.R) {
-        t.su[s] = vu;
-        if (HAS_F) t.sf[s] = vf;
-        t.scx[s] = vx;
-      }
-      t.scz[s] = vz;
-    }
-  }
-}
-
-// tile_load for k_up with the prolongation folded in: su = u + (P e) at
-// every point the bilinear prolongation reaches (prolongate_kernel,
-// odeSolver.cpp:1017-1043; the same single addition per point as a separate
-// pass over su), so the coarse correction's loads are in flight with the
-// tile's and the pass and its barrier go away.
-template <typename T, bool HAS_F, bool FULL_TOP>
-__device__ inline void tile_load_up(Tile<T> &t, const T *__restrict__ u,
-                                    const T *__restrict__ f,
-                                    const T *__restrict__ cx,
-                                    const T *__restrict__ cz, int N, int K, int P,
-                                    const T *__restrict__ e, int Nc, int Kc,
-                                    int Pc) {
-  const int tx = threadIdx.x, ty = threadIdx.y;
-  for (int lr = ty; lr <= t.R; lr += blockDim.y) {
-    const int j = t.j0 + lr;
-    const bool jin = (j >= 0) && (j < N) && lr < t.R;
-    const bool jinz = (j >= 0) && (j <= N);
-    const size_t grow = (size_t)j * P;
-    // the coarse rows of the point's correction (row cj + 1 only for odd j)
-    const int cj = j >> 1, dj = j & 1;
-    const bool jp = jin && (cj < Nc - 1 || (FULL_TOP && cj == Nc - 1 && dj == 0));
-    const size_t r0 = (size_t)(jp ? cj : 0) * Pc;
-    const size_t r1 = (size_t)(jp ? cj + dj : 0) * Pc;
-#pragma unroll
-    for (int cc = 0; cc < 5; cc++) {
-      const int lc = tx + TX_THREADS * cc;
-      if (lc >= t.C) break;
-      const int k = t.k0 + lc;
-      const bool kin = (k >= 0) && (k < K);
-      const int s = lr * t.LS + lc;
-      const bool in = jin && kin;
-      const int ck = k >> 1, dk = k & 1;
-      const bool pr = jp && kin && ck < Kc - 1;
-      T vu = T(0), vf = T(0), vx = T(0), vz = T(0);
-      T e00 = T(0), e01 = T(0), e10 = T(0), e11 = T(0);
-      if (in) vu = u[grow + k];
-      if (HAS_F && in) vf = f[grow + k];
-      if (in) vx = cx[grow + k];
-      if (jinz && kin) vz = cz[grow + k];
-      if (pr) {
-        e00 = e[r0 + ck];
-        e01 = e[r0 + ck + dk];
-        e10 = e[r1 + ck];
-        e11 = e[r1 + ck + dk];
-      }
-      if (lr < t.R) {
-        if (pr) {
-          T add;
-          if (!dj && !dk)
-            add = e00;
-          else if (!dj)
-            add = T(0.5) * (e00 + e01);
-          else if (!dk)
-            add = T(0.5) * (e00 + e10);
-          else
-            add = T(0.25) * (e00 + e01 + e10 + e11);
-          vu += add;
-        }
-        t.su[s] = vu;
-        if (HAS_F) t.sf[s] = vf;
-        t.scx[s] = vx;
-      }
-      t.scz[s] = vz;
-    }
-  }
-}
-
 // Points (r, c) of a rows x cols window dealt to the workgroup's threads in
 // turn, row by row: thread tid takes points tid, tid + nthr, ...  Every lane
 // has a point even where a row is shorter than a wavefront (a 66-point tile
@@ -180,6 +76,135 @@ struct FlatWalk {
     }
   }
 };
+
+// Loads rows [0, R] (row R: the north faces in scz only) x columns [0, C)
+// of the tile, the points dealt out by FlatWalk, LOAD_BATCH points per
+// thread at a time: every global load of a batch is issued, at clamped
+// in-range addresses, before the batch's LDS stores; values outside the
+// grid are stored as zero.  With UP (k_up) the prolongation is folded in:
+// su = u + (P e) at every point the bilinear prolongation reaches
+// (prolongate_kernel, odeSolver.cpp:1017-1043; the same single addition per
+// point as a separate pass over su would make), so the coarse correction's
+// loads are in flight with the tile's.
+template <typename T, bool HAS_F, bool UP, bool FULL_TOP, int LOAD_BATCH>
+__device__ inline void tile_load_batches(Tile<T> &t, const T *__restrict__ u,
+                                         const T *__restrict__ f,
+                                         const T *__restrict__ cx,
+                                         const T *__restrict__ cz, int N, int K,
+                                         int P, bool zero_u,
+                                         const T *__restrict__ e, int Nc, int Kc,
+                                         int Pc) {
+  const int rows = t.R + 1;
+  FlatWalk w(t.C);
+  while (w.r < rows) {
+    int lr[LOAD_BATCH], lc[LOAD_BATCH];
+#pragma unroll
+    for (int i = 0; i < LOAD_BATCH; i++) {
+      lr[i] = w.r;
+      lc[i] = w.c;
+      w.next();
+    }
+    T vu[LOAD_BATCH], vf[LOAD_BATCH], vx[LOAD_BATCH], vz[LOAD_BATCH];
+    T e00[LOAD_BATCH], e01[LOAD_BATCH], e10[LOAD_BATCH], e11[LOAD_BATCH];
+#pragma unroll
+    for (int i = 0; i < LOAD_BATCH; i++) {
+      const int j = t.j0 + lr[i], k = t.k0 + lc[i];
+      const int jc = min(max(j, 0), N - 1), jz = min(max(j, 0), N);
+      const int kc = min(max(k, 0), K - 1);
+      const size_t g = (size_t)jc * P + kc;
+      vu[i] = u[g];
+      if (HAS_F) vf[i] = f[g];
+      vx[i] = cx[g];
+      vz[i] = cz[(size_t)jz * P + kc];
+      if (UP) {
+        // coarse point (cj, ck) and its neighbours; clamped like the fine
+        // point, so every address is inside the coarse grid
+        const int cj = min(jc >> 1, Nc - 1), ck = min(kc >> 1, Kc - 1);
+        const int cj1 = min(cj + (jc & 1), Nc - 1), ck1 = min(ck + (kc & 1), Kc - 1);
+        e00[i] = e[(size_t)cj * Pc + ck];
+        e01[i] = e[(size_t)cj * Pc + ck1];
+        e10[i] = e[(size_t)cj1 * Pc + ck];
+        e11[i] = e[(size_t)cj1 * Pc + ck1];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < LOAD_BATCH; i++) {
+      if (lr[i] >= rows) continue;
+      const int j = t.j0 + lr[i], k = t.k0 + lc[i];
+      const bool kin = (k >= 0) && (k < K);
+      const bool in = kin && (j >= 0) && (j < N) && lr[i] < t.R;
+      const bool inz = kin && (j >= 0) && (j <= N);
+      const int s = lr[i] * t.LS + lc[i];
+      if (lr[i] < t.R) {
+        T v = (in && !zero_u) ? vu[i] : T(0);
+        if (UP && in) {
+          const int cj = j >> 1, dj = j & 1, ck = k >> 1, dk = k & 1;
+          const bool pr = (cj < Nc - 1 || (FULL_TOP && cj == Nc - 1 && dj == 0)) &&
+                          ck < Kc - 1;
+          if (pr) {
+            T add;
+            if (!dj && !dk)
+              add = e00[i];
+            else if (!dj)
+              add = T(0.5) * (e00[i] + e01[i]);
+            else if (!dk)
+              add = T(0.5) * (e00[i] + e10[i]);
+            else
+              add = T(0.25) * (e00[i] + e01[i] + e10[i] + e11[i]);
+            v += add;
+          }
+        }
+        t.su[s] = v;
+        if (HAS_F) t.sf[s] = in ? vf[i] : T(0);
+        t.scx[s] = in ? vx[i] : T(0);
+      }
+      t.scz[s] = inz ? vz[i] : T(0);
+    }
+  }
+}
+
+// The batch size follows the points per thread (uniform over the
+// workgroup): small tiles do not issue loads for empty batch slots.
+template <typename T, bool HAS_F, bool UP, bool FULL_TOP>
+__device__ inline void tile_load_impl(Tile<T> &t, const T *__restrict__ u,
+                                      const T *__restrict__ f,
+                                      const T *__restrict__ cx,
+                                      const T *__restrict__ cz, int N, int K,
+                                      int P, bool zero_u, const T *__restrict__ e,
+                                      int Nc, int Kc, int Pc) {
+  const int nthr = blockDim.x * blockDim.y;
+  const int per = ((t.R + 1) * t.C + nthr - 1) / nthr;
+  if (per >= 4)
+    tile_load_batches<T, HAS_F, UP, FULL_TOP, 4>(t, u, f, cx, cz, N, K, P, zero_u, e,
+                                                 Nc, Kc, Pc);
+  else if (per >= 2)
+    tile_load_batches<T, HAS_F, UP, FULL_TOP, 2>(t, u, f, cx, cz, N, K, P, zero_u, e,
+                                                 Nc, Kc, Pc);
+  else
+    tile_load_batches<T, HAS_F, UP, FULL_TOP, 1>(t, u, f, cx, cz, N, K, P, zero_u, e,
+                                                 Nc, Kc, Pc);
+}
+
+template <typename T, bool HAS_F>
+__device__ inline void tile_load(Tile<T> &t, const T *__restrict__ u,
+                                 const T *__restrict__ f,
+                                 const T *__restrict__ cx,
+                                 const T *__restrict__ cz, int N, int K, int P,
+                                 bool zero_u) {
+  tile_load_impl<T, HAS_F, false, false>(t, u, f, cx, cz, N, K, P, zero_u, nullptr,
+                                         0, 0, 0);
+}
+
+template <typename T, bool HAS_F, bool FULL_TOP>
+__device__ inline void tile_load_up(Tile<T> &t, const T *__restrict__ u,
+                                    const T *__restrict__ f,
+                                    const T *__restrict__ cx,
+                                    const T *__restrict__ cz, int N, int K, int P,
+                                    const T *__restrict__ e, int Nc, int Kc,
+                                    int Pc) {
+  tile_load_impl<T, HAS_F, true, FULL_TOP>(t, u, f, cx, cz, N, K, P, false, e, Nc,
+                                           Kc, Pc);
+}
 
 // Half-sweep `colour` over the local window [h, R-h) x [h, C-h): the half
 // points of one colour in each row, all rows, dealt out by FlatWalk.  The
