@@ -10,8 +10,8 @@ default run warms up W steps from t = 0 and times the next K accepted steps
 (W + K = 3020: the workload's 3000-step run, which spans its first seismic
 events).  Inputs are synthetic (generated, no data files).
 
-Modes: fast64 (the headline: FP64 V(2,2) fast cycle on the tiled LDS
-kernels, exact coarsest solve), ir32 (FP32 correction cycles in FP64
+Modes: fast64 (the headline: FP64 V(1,3) fast cycle on the tiled LDS
+kernels, direct coarsest solve), ir32 (FP32 correction cycles in FP64
 iterative refinement) and ref64 (the reference's own V-cycle: 32 sweeps per
 level, FP64).  One JSON line goes to stdout; progress goes to stderr.
 
@@ -45,7 +45,7 @@ METRIC = ("RK43 steps/s at 1025² to 1e-10 residual; V-cycles/solve, "
           "learned vs plain")
 
 # The fast cycle: one pre- and three post-smoothing red/black sweeps,
-# over-relaxed (1.15), and the exact 17^2 coarsest solve.  Swept over
+# over-relaxed (1.15), and the direct 17^2 coarsest solve.  Swept over
 # V(1,1)..V(4,4) and relaxations 1.0..1.4 at 257^2..2049^2; this shape and
 # factor were best at every grid size in FP64 and FP32-IR.
 MODES = {
@@ -724,7 +724,7 @@ def main():
         "config": {"workload": "basin_1025", "grid": "1025x1025", "levels": 7,
                    "residual": "1e-10 relative", "mode": args.mode,
                    "cycle": ("reference V(32,32)" if args.mode == "ref64" else
-                             "V(1,3), red/black relaxation 1.15, exact 17x17 "
+                             "V(1,3), red/black relaxation 1.15, direct 17x17 "
                              "coarsest solve"),
                    "initial_guess": "previous solution",
                    "tile_shapes": TUNED.get(args.mode),

@@ -454,7 +454,7 @@ void orc_mg_destroy(orc_mg *mg) {
   free(mg);
 }
 
-/* The engine's fast cycle may solve the coarsest level exactly (no
+/* The engine's fast cycle may solve the coarsest level directly (no
  * reference counterpart): B = DH^2 L assembled on the unknowns
  * j*(K-2) + k-1 from the same coefficients the sweeps use, inverted by
  * Gauss-Jordan elimination without pivoting, applied as e = B^{-1}(DH^2 f)
@@ -509,6 +509,9 @@ static void orc_coarse_inverse(orc_mg *mg) {
       }
     }
   }
+  /* the engine stores the inverse in single precision (the solve's
+   * products and sums stay in double) */
+  for (long i = 0; i < n * n; i++) X[i] = (double)(float)X[i];
   free(B);
   free(mg->cinv);
   mg->cinv = X;

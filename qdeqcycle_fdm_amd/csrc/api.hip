@@ -80,7 +80,6 @@ static void free_ctx(qdeq_ctx *c) {
     (void)hipFree(c->s[l].cx); (void)hipFree(c->s[l].cz);
   }
   (void)hipFree(c->mu_equi);
-  (void)hipFree(c->cinv64);
   (void)hipFree(c->cinv32);
   (void)hipFree(c->partial);
   (void)hipFree(c->gate);

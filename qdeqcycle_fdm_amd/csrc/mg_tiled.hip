@@ -405,7 +405,7 @@ struct TailArgs {
   void *e_out;          // correction of the entry level (global)
   int nu_pre, nu_post, nu_coarse;
   double relax;
-  const void *minv;     // exact coarsest solve (column-major inverse) or null
+  const void *minv;     // direct coarsest solve (column-major FP32 inverse) or null
   int cn;               // coarsest unknowns for minv
   const int *gate;      // skip flag of a batched solve, or null
 };
@@ -481,7 +481,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailArgs a) {
   T *Rs = tail_end(base, a);  // residual scratch (entry level size)
   T *Gs = Rs + (a.lv[0].N * a.lv[0].K > a.cn ? a.lv[0].N * a.lv[0].K : a.cn);
   T *Ps = Gs + a.cn;
-  // Gs: right-hand side DH^2 f of the exact coarsest solve; Ps: its 1024
+  // Gs: right-hand side DH^2 f of the direct coarsest solve; Ps: its 1024
   // partial sums
   const int tid = threadIdx.x;
   const T relax = (T)a.relax;
@@ -547,7 +547,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailArgs a) {
       if (a.minv) {
         // e = B^{-1} (DH^2 f), rows summed in column order; Gs was filled by
         // the restriction onto this level unless it is the entry level
-        const T *minv = reinterpret_cast<const T *>(a.minv);
+        const float *minv = reinterpret_cast<const float *>(a.minv);
         const int W = L.K - 2;
         if (l == 0) {
           for (int i = tid; i < a.cn; i += blockDim.x) {
@@ -583,7 +583,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailArgs a) {
     __syncthreads();
     const TailLevel &C = a.lv[l + 1];
     const TailBufs<T> Bc = tail_bufs(base, a, l + 1);
-    // onto the coarsest level of an exact solve: also its right-hand side
+    // onto the coarsest level of a direct solve: also its right-hand side
     const bool direct = a.minv && l + 1 == a.nlev - 1;
     const T DH2c = (T)C.DH2;
     for (int idx = tid; idx < C.N * C.K; idx += blockDim.x) {
@@ -668,7 +668,7 @@ static size_t tail_lds_bytes(const qdeq_ctx *c, int lt) {
   for (int l = lt; l < c->L; l++) n += tail_level_words(c->lv[l].N, c->lv[l].K);
   const size_t nr = (size_t)c->lv[lt].N * c->lv[lt].K;
   n += nr > (size_t)c->coarse_n ? nr : (size_t)c->coarse_n;
-  n += c->coarse_n;  // right-hand side of the exact coarsest solve
+  n += c->coarse_n;  // right-hand side of the direct coarsest solve
   n += 1024;         // its partial sums
   return n * sizeof(T);
 }
@@ -770,9 +770,7 @@ static void launch_tail(qdeq_ctx *c, LevelBufs<T> *B, int lt) {
   a.nu_coarse = c->cfg.n_coarse;
   a.relax = c->cfg.relax;
   a.cn = c->coarse_n;
-  a.minv = c->coarse_n > 0 ? (sizeof(T) == 8 ? (const void *)c->cinv64
-                                             : (const void *)c->cinv32)
-                           : nullptr;
+  a.minv = c->coarse_n > 0 ? (const void *)c->cinv32 : nullptr;
   a.gate = c->gate_ptr;
   hipLaunchKernelGGL((k_tail<T, EXACT, true>), dim3(1), dim3(TAIL_THREADS),
                      tail_lds_bytes<T>(c, lt), c->stream, a);

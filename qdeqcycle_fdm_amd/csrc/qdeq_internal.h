@@ -147,9 +147,9 @@ struct qdeq_ctx {
   bool tuned = false;
   qdeq::TileShape tdown[QDEQ_MAX_LEVELS] = {}, tup[QDEQ_MAX_LEVELS] = {};
   size_t lds_max = 65536;  // dynamic LDS per workgroup (queried at creation)
-  // exact coarsest-level solve of the fast cycle (n_coarse == 0): inverse of
-  // the coarsest operator, column-major, FP64 and FP32 copies
-  double *cinv64 = nullptr;
+  // direct coarsest-level solve of the fast cycle (n_coarse == 0): inverse of
+  // the coarsest operator, column-major, stored in FP32 (products and sums
+  // in the solve's own precision)
   float *cinv32 = nullptr;
   int coarse_n = 0;
   // HIP-event timing of one kernel of the fast cycle (0 = off; see
@@ -275,7 +275,7 @@ __host__ __device__ inline int coarse_chunk_len(int n) {
 // e[row] (rows < n, all threads of the block) = sum_col minvT[col*n+row] g[col]
 // in the chunked order; part: >= 1024 LDS elements; ends with a barrier.
 template <typename T>
-__device__ inline void coarse_matvec(const T *__restrict__ minvT, const T *g,
+__device__ inline void coarse_matvec(const float *__restrict__ minvT, const T *g,
                                      T *part, int n, T *out, int W, int K,
                                      size_t pitch) {
   const int Q = coarse_chunks(n), C = coarse_chunk_len(n);
@@ -285,7 +285,7 @@ __device__ inline void coarse_matvec(const T *__restrict__ minvT, const T *g,
     T acc = T(0);
 #pragma unroll 8
     for (int col = c0; col < c1; col++)
-      acc = acc + minvT[(size_t)col * n + row] * g[col];
+      acc = acc + (T)minvT[(size_t)col * n + row] * g[col];
     part[t] = acc;
   }
   __syncthreads();
