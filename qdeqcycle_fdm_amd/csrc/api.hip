@@ -235,11 +235,12 @@ int qdeq_create(const qdeq_config *cfg, const double *mu,
     free_ctx(c);
     return QDEQ_ERR_HIP;
   }
-  // QDEQ_DIRECT_LAUNCH=1: V-cycles are launched kernel by kernel instead of
-  // as the captured graph (same kernels and arguments), for profilers that
-  // cannot trace graph launches
+  // QDEQ_DIRECT_LAUNCH=1 / QDEQ_GRAPH_LAUNCH=1: V-cycles always launched
+  // kernel by kernel / always as the captured graph (same kernels and
+  // arguments); by default direct, graphs inside qdeq_ensemble_step
   const char *dl = std::getenv("QDEQ_DIRECT_LAUNCH");
-  c->direct_launch = dl && dl[0] == '1';
+  const char *gl = std::getenv("QDEQ_GRAPH_LAUNCH");
+  c->launch_env = (dl && dl[0] == '1') ? 1 : ((gl && gl[0] == '1') ? 2 : 0);
   // QDEQ_HOST_START=1: the host reads every solve's start norms and decides
   // (the path FP32-IR solves always take) instead of the device
   const char *hs = std::getenv("QDEQ_HOST_START");
@@ -855,6 +856,8 @@ int qdeq_ensemble_step(qdeq_ctx **ctxs, int n, int nsteps, int *status) {
     return set_error(nullptr, QDEQ_ERR_ARG, "bad argument");
   std::vector<int> done((size_t)n, 0);
   int st_all = QDEQ_OK;
+  for (int i = 0; i < n; i++)
+    if (ctxs[i]) ctxs[i]->in_ensemble = true;
   for (int i = 0; i < n; i++) {
     int st = ctxs[i] ? rk_request(ctxs[i], nsteps) : QDEQ_ERR_ARG;
     if (status) status[i] = st;
@@ -886,6 +889,8 @@ int qdeq_ensemble_step(qdeq_ctx **ctxs, int n, int nsteps, int *status) {
       }
     }
   }
+  for (int i = 0; i < n; i++)
+    if (ctxs[i]) ctxs[i]->in_ensemble = false;
   return st_all;
 }
 

@@ -360,7 +360,7 @@ int guess_before_solve(qdeq_ctx *c) {
     c->harvest_pending = true;
     return QDEQ_OK;
   }
-  if (c->direct_launch) {
+  if (launch_direct(c)) {
     mbox_arm(c, SC_NORM_PRED);
     launch_predict_chain(c);
     QDEQ_HIP(c, hipGetLastError());

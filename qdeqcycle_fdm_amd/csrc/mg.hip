@@ -921,7 +921,7 @@ static int launch_batch(qdeq_ctx *c, int k) {
       QDEQ_HIP(c, hipGetLastError());
       m.sample = true;
       m.sample_ordinal = m.launched;
-    } else if (c->direct_launch) {
+    } else if (launch_direct(c)) {
       c->gate_ptr = c->gate;
       launch_iteration(c);
       c->gate_ptr = nullptr;

@@ -116,7 +116,17 @@ struct qdeq_ctx {
   hipStream_t stream = nullptr;
   hipGraphExec_t g_cycle = nullptr;   // one V-cycle (or IR sweep) + residual norm
   int g_cycle_mode = -1;
-  bool direct_launch = false;  // launch V-cycles kernel by kernel (profiling)
+  // V-cycles (and the learned guess's inference chain) are launched kernel by
+  // kernel or as captured graphs.  Measured at 1025^2: one simulation steps
+  // 3% faster with direct launches (a graph launch costs the GPU more than
+  // the kernels' own launches, and the host keeps ahead of the GPU), while
+  // an ensemble of 64 members driven from one host thread steps 12% faster
+  // with graphs (there the host's launch work is the bound).  launch_env:
+  // 0 = that choice, 1 = always direct (QDEQ_DIRECT_LAUNCH=1, for profilers
+  // that cannot trace graph launches), 2 = always graphs
+  // (QDEQ_GRAPH_LAUNCH=1); in_ensemble is set while qdeq_ensemble_step runs.
+  int launch_env = 0;
+  bool in_ensemble = false;
   bool host_start = false;     // start decisions on the host (see mg_begin)
   int last_it = 2;             // V-cycles of the last solve (first batch size)
   // RK43 solves: the stage's key (stage s, +8 for the solve after an
@@ -299,6 +309,10 @@ __device__ inline void coarse_matvec(const float *__restrict__ minvT, const T *g
 }
 
 int build_coarse_inverse(qdeq_ctx *c);
+
+inline bool launch_direct(const qdeq_ctx *c) {
+  return c->launch_env == 1 || (c->launch_env == 0 && !c->in_ensemble);
+}
 void launch_coarse_direct_fp64(qdeq_ctx *c, double *u, const double *f);
 void launch_coarse_direct_fp32(qdeq_ctx *c, float *u, const float *f);
 

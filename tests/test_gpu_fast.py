@@ -241,6 +241,33 @@ def test_kernel_timing_does_not_change_results():
             np.testing.assert_array_equal(st[k], out[0][1][k])
 
 
+@pytest.mark.parametrize("precision", [config.PREC_FP64, config.PREC_FP32_IR])
+def test_graph_and_direct_launches_are_bit_identical(precision, monkeypatch):
+    """One simulation launches its V-cycles kernel by kernel by default; with
+    QDEQ_GRAPH_LAUNCH=1 every V-cycle is a replay of the captured graph (many
+    replays per run, with the host's own launches between them).  Both runs
+    and the default must agree bit for bit."""
+    pb = config.make_bp1_problem(257, basin=True, levels=5,
+                                 tol_mode=config.TOL_REL, tol=1e-10)
+    me = oracle.mu_equi(pb.mu)
+    out = []
+    for env in (None, "QDEQ_GRAPH_LAUNCH", "QDEQ_DIRECT_LAUNCH"):
+        monkeypatch.delenv("QDEQ_GRAPH_LAUNCH", raising=False)
+        monkeypatch.delenv("QDEQ_DIRECT_LAUNCH", raising=False)
+        if env:
+            monkeypatch.setenv(env, "1")
+        with lib.Engine(pb, me, **{**FAST, "precision": precision}) as eng:
+            eng.set_tiles(True, 64, 16)
+            eng.rk43_setup()
+            info = eng.rk43_step(40)
+            out.append((info, eng.state()))
+    assert out[0][0]["vcycles"] > 40
+    for info, st in out[1:]:
+        assert info["t"] == out[0][0]["t"] and info["vcycles"] == out[0][0]["vcycles"]
+        for k in ("slip", "psi", "U"):
+            np.testing.assert_array_equal(st[k], out[0][1][k])
+
+
 def test_ir_and_fp64_at_2049():
     """The mixed-precision workload's grid: 2049^2, 8 levels.  FP32 V-cycles
     in FP64 refinement and the FP64 cycle (bit-checked against the oracle at
