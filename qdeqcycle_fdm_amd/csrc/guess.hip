@@ -341,7 +341,7 @@ static void launch_predict_chain(qdeq_ctx *c) {
   const dim3 grd((d.P + 255) / 256, d.N);
   hipLaunchKernelGGL(k_guess_apply, grd, blk, 0, st, c->d[0].u, c->wr, c->gy, c->gx,
                      c->gs, d.N, d.K, d.P, g->f, g->nc, c->d[0].r);
-  launch_resnorm_of(c, c->d[0].r, 2 * 1024, SC_NORM_PRED);
+  launch_resnorm_of(c, c->d[0].r, 2 * kNormCap, SC_NORM_PRED);
 }
 
 int guess_before_solve(qdeq_ctx *c) {

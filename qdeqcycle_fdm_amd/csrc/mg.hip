@@ -19,7 +19,10 @@
 namespace qdeq {
 
 constexpr int BX = 64, BY = 4;          // 256-thread tiles, 64-wide rows
-constexpr int NORM_BLOCKS = 1024, NORM_THREADS = 256;
+// NORM_BLOCKS: capacity of the row-partial layout; a launch uses
+// norm_blocks(c) = min(N, NORM_BLOCKS) blocks, one row each up to 2048 rows
+// (one block with twice the rows of the others would set the time)
+constexpr int NORM_BLOCKS = kNormCap, NORM_THREADS = 256;
 
 // ---------------------------------------------------------------------------
 // Kernels
@@ -297,9 +300,9 @@ __global__ __launch_bounds__(NORM_THREADS) void k_norm_gate(
 // columns from u), of a candidate start (when cand is given) and of the
 // boundary data, each over the same decomposition and in the same order as
 // k_resnorm_partial and k_bdres_partial (8 blocks), so every norm has the
-// same bits as when those kernels compute it.  partial[0..NORM_BLOCKS):
+// same bits as when those kernels compute it.  partial[0..nb):
 // plain start; partial[NORM_BLOCKS..+8): boundary data;
-// partial[2*NORM_BLOCKS..): candidate.
+// partial[2*NORM_BLOCKS..+nb): candidate (nb = gridDim.x).
 constexpr int BDRES_BLOCKS = 8;
 
 template <bool CAND>
@@ -379,17 +382,17 @@ __global__ __launch_bounds__(NORM_THREADS) void k_start_partials(
 // used, gate[5] candidate given but not used.  Counts as one step of the
 // mailbox sequence.
 __global__ __launch_bounds__(NORM_THREADS) void k_start_gate(
-    const double *__restrict__ partial, int cand, int rel, double tol, int cap,
+    const double *__restrict__ partial, int nb, int cand, int rel, double tol, int cap,
     double *__restrict__ scal, double *__restrict__ mb, int *__restrict__ gate) {
   // the three sums of k_norm_final's order, in one tree
   __shared__ double sh[3][NORM_THREADS];
   const int tid = threadIdx.x;
   double a = 0.0, b = 0.0, cc = 0.0;
-  for (int i = tid; i < NORM_BLOCKS; i += NORM_THREADS) a += partial[i];
+  for (int i = tid; i < nb; i += NORM_THREADS) a += partial[i];
   if (rel)
     for (int i = tid; i < BDRES_BLOCKS; i += NORM_THREADS) b += partial[NORM_BLOCKS + i];
   if (cand)
-    for (int i = tid; i < NORM_BLOCKS; i += NORM_THREADS) cc += partial[2 * NORM_BLOCKS + i];
+    for (int i = tid; i < nb; i += NORM_THREADS) cc += partial[2 * NORM_BLOCKS + i];
   sh[0][tid] = a;
   sh[1][tid] = b;
   sh[2][tid] = cc;
@@ -599,28 +602,29 @@ static void launch_resnorm(qdeq_ctx *c, bool store32) {
   const LevelDims &d = c->lv[0];
   const double DH2 = d.DH * d.DH;
   if (store32)
-    hipLaunchKernelGGL((k_resnorm_partial<true>), dim3(NORM_BLOCKS),
+    hipLaunchKernelGGL((k_resnorm_partial<true>), dim3(norm_blocks(c)),
                        dim3(NORM_THREADS), 0, c->stream, c->d[0].u, c->d[0].cx,
                        c->d[0].cz, d.N, d.K, d.P, DH2, c->partial, c->s[0].f,
                        c->s[0].u, (const int *)c->gate_ptr);
   else
-    hipLaunchKernelGGL((k_resnorm_partial<false>), dim3(NORM_BLOCKS),
+    hipLaunchKernelGGL((k_resnorm_partial<false>), dim3(norm_blocks(c)),
                        dim3(NORM_THREADS), 0, c->stream, c->d[0].u, c->d[0].cx,
                        c->d[0].cz, d.N, d.K, d.P, DH2, c->partial,
                        (float *)nullptr, (float *)nullptr, (const int *)c->gate_ptr);
-  launch_norm_final(c, NORM_BLOCKS);
+  launch_norm_final(c, norm_blocks(c));
 }
 
 // Residual norm of an arbitrary level-0 displacement u (same operator and
 // summation as the stop test) into scal[slot]; partials at partial + off.
 void launch_resnorm_of(qdeq_ctx *c, const double *u, int off, int slot) {
   const LevelDims &d = c->lv[0];
-  hipLaunchKernelGGL((k_resnorm_partial<false>), dim3(NORM_BLOCKS),
+  hipLaunchKernelGGL((k_resnorm_partial<false>), dim3(norm_blocks(c)),
                      dim3(NORM_THREADS), 0, c->stream, u, c->d[0].cx, c->d[0].cz,
                      d.N, d.K, d.P, d.DH * d.DH, c->partial + off,
                      (float *)nullptr, (float *)nullptr, (const int *)nullptr);
   hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(NORM_THREADS), 0, c->stream,
-                     c->partial + off, NORM_BLOCKS, c->scal + slot, c->mbox_dev + slot);
+                     c->partial + off, norm_blocks(c), c->scal + slot,
+                     c->mbox_dev + slot);
 }
 
 static void launch_bdres(qdeq_ctx *c) {
@@ -811,15 +815,15 @@ int mg_begin(qdeq_ctx *c) {
     const double *plain = in_u ? c->plain_interior : c->d[0].u;
     const double *cand = !m.cand ? nullptr : (in_u ? c->d[0].u : c->d[0].r);
     if (cand)
-      hipLaunchKernelGGL(k_start_partials<true>, dim3(NORM_BLOCKS), dim3(NORM_THREADS), 0,
+      hipLaunchKernelGGL(k_start_partials<true>, dim3(norm_blocks(c)), dim3(NORM_THREADS), 0,
                          c->stream, plain, c->d[0].u, cand, c->d[0].cx, c->d[0].cz,
                          d.N, d.K, d.P, d.DH * d.DH, rel, c->partial);
     else
-      hipLaunchKernelGGL(k_start_partials<false>, dim3(NORM_BLOCKS), dim3(NORM_THREADS), 0,
+      hipLaunchKernelGGL(k_start_partials<false>, dim3(norm_blocks(c)), dim3(NORM_THREADS), 0,
                          c->stream, plain, c->d[0].u, cand, c->d[0].cx, c->d[0].cz,
                          d.N, d.K, d.P, d.DH * d.DH, rel, c->partial);
     hipLaunchKernelGGL(k_start_gate, dim3(1), dim3(NORM_THREADS), 0, c->stream,
-                       c->partial, m.cand ? 1 : 0, rel, c->cfg.tol,
+                       c->partial, norm_blocks(c), m.cand ? 1 : 0, rel, c->cfg.tol,
                        c->cfg.max_vcycles, c->scal, c->mbox_dev, c->gate);
     if (in_u) {
       // the plain start back over a candidate that was not chosen

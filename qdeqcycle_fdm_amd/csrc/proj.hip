@@ -339,7 +339,7 @@ int proj_before_solve(qdeq_ctx *c) {
                      c->proj_u, (size_t)d.N * d.P, c->proj_r, sl, c->proj_coef,
                      d.N, d.K, d.P, dev ? c->d[0].u : c->d[0].r, dev ? 0 : 1,
                      c->proj_inject);
-  if (!dev) launch_resnorm_of(c, c->d[0].r, 2 * 1024, SC_NORM_PRED);
+  if (!dev) launch_resnorm_of(c, c->d[0].r, 2 * kNormCap, SC_NORM_PRED);
   QDEQ_HIP(c, hipGetLastError());
   c->cand_pending = true;
   c->cand_in_u = dev;

@@ -310,6 +310,14 @@ __device__ inline void coarse_matvec(const float *__restrict__ minvT, const T *g
 
 int build_coarse_inverse(qdeq_ctx *c);
 
+// Row-partial layout of the level-0 residual norms (mg.hip): one block per
+// row, at most kNormCap blocks; plain norms at partial[0..), boundary data
+// at partial[kNormCap..), a candidate start's norm at partial[2 kNormCap..).
+constexpr int kNormCap = 2048;
+inline int norm_blocks(const qdeq_ctx *c) {
+  return c->lv[0].N < kNormCap ? c->lv[0].N : kNormCap;
+}
+
 inline bool launch_direct(const qdeq_ctx *c) {
   return c->launch_env == 1 || (c->launch_env == 0 && !c->in_ensemble);
 }
