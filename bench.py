@@ -352,14 +352,16 @@ def run_mixed_2049(n_events=2, tail_steps=200):
     """Mixed-precision workload: the basin model on 2049^2 (8 levels) with
     FP32 V-cycles in FP64 refinement, against the FP64 cycle on the same
     grid (the precision error) and the 1025^2 FP64 run (the resolution
-    difference).  Every run starts at t = 0.  The 1025^2 FP64 run and the
-    2049^2 FP32-IR run each step until n_events events (max V > 1e-3 m/s) have
-    ended and then tail_steps more, into the quiet phase; the 2049^2 FP64 run
-    stops exactly at the IR run's final time T.  Reported: rates, the slip and
-    log10 V differences between the two 2049^2 runs on the fault at T, and
-    event onsets and recurrence intervals of all three (the two grids resolve
-    the nucleation differently, so they are compared through the event
-    catalogue, not through states at one time)."""
+    difference).  Every run starts at t = 0.  The 1025^2 FP64 run, the
+    2049^2 FP32-IR run and the 2049^2 FP64 run with the projection start
+    each step until n_events events (max V > 1e-3 m/s) have ended and then
+    tail_steps more, into the quiet phase; the plain 2049^2 FP64 run stops
+    exactly at the IR run's final time T.  Reported: rates, the slip and
+    log10 V differences between the IR and plain FP64 2049^2 runs on the
+    fault at T, and event onsets and recurrence intervals of all runs (the
+    two grids resolve the nucleation differently, and the projection run's
+    last event may fall on either side of T, so those are compared through
+    the event catalogue, not through states at one time)."""
     import numpy as np
     from qdeqcycle_fdm_amd import config, lib
 
@@ -403,7 +405,7 @@ def run_mixed_2049(n_events=2, tail_steps=200):
     ir = run(2049, "ir32")
     T = ir["info"]["t"]
     f64 = run(2049, "fast64", tmax=T)
-    f64p = run(2049, "fast64", tmax=T, proj=6)
+    f64p = run(2049, "fast64", proj=6)
 
     def catalogue(a, b):
         ea = [e["t_onset"] for e in a["events"]]
@@ -434,7 +436,7 @@ def run_mixed_2049(n_events=2, tail_steps=200):
                            "2049_fp64": rate(f64),
                            "2049_fp64_projection_6_slots": rate(f64p)},
                 ir32_vs_fp64_2049=cmp(ir, f64),
-                fp64_projection_vs_fp64_2049=cmp(f64p, f64),
+                fp64_projection_vs_fp64_2049=catalogue(f64p, f64),
                 ir32_2049_vs_fp64_1025=catalogue(ir, ref))
 
 
