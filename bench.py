@@ -522,9 +522,9 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU "
-              "path (independent simulations shard without communication; see "
-              "DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU only: the engine has no multi-GPU "
+              "path (independent simulations shard without communication, one "
+              "process per GPU). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     if args.ensemble and "GPU_MAX_HW_QUEUES" not in os.environ:
