@@ -276,6 +276,16 @@ __device__ inline Tile<T> make_tile(unsigned char *smem, int TX, int TY, int H,
   return t;
 }
 
+// Without a right-hand side (level 0 of the FP64 cycle) k_down keeps the
+// residuals of its (TY+2) x (TX+2) window in registers, up to RES_REGS per
+// thread, and then writes them over scx, which the restriction no longer
+// needs: the tile then takes three arrays of LDS instead of four, which
+// lets two workgroups share a CU.  Larger windows use a scratch array.
+constexpr int RES_REGS = 8;
+__host__ __device__ inline bool res_in_regs(int TX, int TY, int threads) {
+  return (TX + 2) * (TY + 2) <= RES_REGS * threads;
+}
+
 // Pre-smoothing + residual + restriction (levels above the tail).
 template <typename T, bool HAS_F, bool EXACT>
 __global__ __launch_bounds__(TILE_THREADS) void k_down(
@@ -291,17 +301,49 @@ __global__ __launch_bounds__(TILE_THREADS) void k_down(
   __syncthreads();
   tile_sweeps<T, HAS_F, EXACT>(t, nu, N, K, DH2, relax);
   // residual on the interior plus one ring; it replaces f (read at the same
-  // point only) or goes to the scratch block after scz
-  T *sr = HAS_F ? t.sf : t.scz + (t.R + 1) * t.LS;
+  // point only), or goes over scx (after a barrier, from registers), or to
+  // the scratch block after scz
   const int h = H - 1;
-  for (FlatWalk w(t.C - 2 * h); w.r < t.R - 2 * h; w.next()) {
-    const int lr = h + w.r, lc = h + w.c;
-    const int j = t.j0 + lr;
-    const int k = t.k0 + lc;
-    T r = T(0);
-    if (j >= 0 && j < N && k >= 1 && k <= K - 2)
-      r = tile_residual<T, HAS_F>(t, lr, lc, j, N, DH2);
-    sr[lr * t.LS + lc] = r;
+  const int rrows = t.R - 2 * h, rcols = t.C - 2 * h;
+  T *sr;
+  if (!HAS_F && res_in_regs(TX, TY, blockDim.x * blockDim.y)) {
+    T rv[RES_REGS];
+    {
+      FlatWalk w(rcols);
+#pragma unroll
+      for (int i = 0; i < RES_REGS; i++) {
+        rv[i] = T(0);
+        if (w.r < rrows) {
+          const int lr = h + w.r, lc = h + w.c;
+          const int j = t.j0 + lr;
+          const int k = t.k0 + lc;
+          if (j >= 0 && j < N && k >= 1 && k <= K - 2)
+            rv[i] = tile_residual<T, HAS_F>(t, lr, lc, j, N, DH2);
+        }
+        w.next();
+      }
+    }
+    __syncthreads();
+    sr = t.scx;
+    {
+      FlatWalk w(rcols);
+#pragma unroll
+      for (int i = 0; i < RES_REGS; i++) {
+        if (w.r < rrows) sr[(h + w.r) * t.LS + h + w.c] = rv[i];
+        w.next();
+      }
+    }
+  } else {
+    sr = HAS_F ? t.sf : t.scz + (t.R + 1) * t.LS;
+    for (FlatWalk w(rcols); w.r < rrows; w.next()) {
+      const int lr = h + w.r, lc = h + w.c;
+      const int j = t.j0 + lr;
+      const int k = t.k0 + lc;
+      T r = T(0);
+      if (j >= 0 && j < N && k >= 1 && k <= K - 2)
+        r = tile_residual<T, HAS_F>(t, lr, lc, j, N, DH2);
+      sr[lr * t.LS + lc] = r;
+    }
   }
   __syncthreads();
   // restriction for the coarse points inside the tile interior
@@ -645,10 +687,11 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_tail(TailArgs a) {
 // ---------------------------------------------------------------------------
 
 static size_t tile_lds_bytes(int TX, int TY, int H, bool has_f, size_t esz,
-                             bool down) {
+                             bool down, int threads) {
   const size_t R = TY + 2 * H, C = TX + 2 * H;
   size_t words = 3 * R * C + C;            // su, scx, scz (+1 row)
-  if (has_f || down) words += R * C;       // sf, or residual scratch
+  if (has_f || (down && !res_in_regs(TX, TY, threads)))
+    words += R * C;                        // sf, or residual scratch
   return words * esz;
 }
 
@@ -696,15 +739,18 @@ static bool shape_fits(const qdeq_ctx *c, const TileShape &s, bool down,
   if (s.tx < 2 || s.ty < 2 || (s.tx & 1) || (s.ty & 1)) return false;
   if (s.tyt < 1 || s.tyt > TY_THREADS || (s.tyt & (s.tyt - 1))) return false;
   if (s.tx + 2 * H > 5 * TX_THREADS) return false;
-  return tile_lds_bytes(s.tx, s.ty, H, has_f, esz, down) + 8192 <= c->lds_max;
+  return tile_lds_bytes(s.tx, s.ty, H, has_f, esz, down, TX_THREADS * s.tyt) + 8192 <=
+         c->lds_max;
 }
 
 // Largest dynamic LDS the tile kernels of this context would request.
 bool tiles_fit(const qdeq_ctx *c, int tx, int ty) {
   const bool fp64 = c->cfg.precision != QDEQ_PREC_FP32_IR;
   const size_t esz = fp64 ? 8 : 4;
-  const size_t down = tile_lds_bytes(tx, ty, 2 * c->cfg.n_pre + 2, true, esz, true);
-  const size_t up = tile_lds_bytes(tx, ty, 2 * c->cfg.n_post + 1, true, esz, false);
+  const size_t down = tile_lds_bytes(tx, ty, 2 * c->cfg.n_pre + 2, true, esz, true,
+                                     TILE_THREADS);
+  const size_t up = tile_lds_bytes(tx, ty, 2 * c->cfg.n_post + 1, true, esz, false,
+                                   TILE_THREADS);
   // tile_load covers at most 5 x 64 columns per row
   if (tx + 2 * (2 * c->cfg.n_pre + 2) > 5 * TX_THREADS) return false;
   if (tx + 2 * (2 * c->cfg.n_post + 1) > 5 * TX_THREADS) return false;
@@ -722,7 +768,8 @@ static void launch_down(qdeq_ctx *c, LevelBufs<T> *B, int l, bool has_f,
   const int H = 2 * nu + 2;
   const dim3 grd((d.K + TX - 1) / TX, (d.N + TY - 1) / TY);
   const dim3 blk(TX_THREADS, sh.tyt);
-  const size_t lds = tile_lds_bytes(TX, TY, H, has_f, sizeof(T), true);
+  const size_t lds = tile_lds_bytes(TX, TY, H, has_f, sizeof(T), true,
+                                    TX_THREADS * sh.tyt);
   const T DH2 = (T)(d.DH * d.DH), relax = (T)c->cfg.relax;
   if (has_f)
     hipLaunchKernelGGL((k_down<T, true, EXACT>), grd, blk, lds,
@@ -747,7 +794,8 @@ static void launch_up_t(qdeq_ctx *c, LevelBufs<T> *B, int l, double *partial,
   if (nblocks) *nblocks = (int)(grd.x * grd.y);
   const int nu = c->cfg.n_post;
   const int H = 2 * nu + (MODE == UP_NORM ? 1 : 0);
-  const size_t lds = tile_lds_bytes(TX, TY, H, HAS_F, sizeof(T), false);
+  const size_t lds = tile_lds_bytes(TX, TY, H, HAS_F, sizeof(T), false,
+                                    TX_THREADS * sh.tyt);
   const T DH2 = (T)(d.DH * d.DH), relax = (T)c->cfg.relax;
   hipLaunchKernelGGL((k_up<T, HAS_F, EXACT, true, MODE>), grd, dim3(TX_THREADS, sh.tyt),
                      lds, c->stream, B[l].r, B[l].u, B[l].f, B[l].cx, B[l].cz,
