@@ -218,6 +218,29 @@ def test_grid_fitted_tile_shapes_give_identical_solutions():
     assert vc == out2[0]
 
 
+def test_level0_residual_in_registers_and_in_scratch_agree():
+    """Level-0 k_down keeps the residuals of its window in registers (up to 8
+    per thread) or, for a larger window per thread, in a scratch array:
+    96x24 tiles on 4 thread rows (98 x 26 > 8 x 256 points) take the scratch
+    path, 96x24 on 16 rows the register path; both solves are bit-identical
+    to the default shapes'."""
+    pb = config.make_bp1_problem(257, basin=True, levels=5,
+                                 tol_mode=config.TOL_REL, tol=1e-10)
+    me = oracle.mu_equi(pb.mu)
+    U = _perturbed(pb, 8)
+    ref, out_ref = _gpu_solve(pb, me, U, True)
+    rest = [((64, 8, 16), (64, 8, 16)), ((32, 4, 8), (32, 4, 8))]
+    for rows in (4, 16):
+        shapes = [((96, 24, rows), (64, 16, 16))] + rest
+        with lib.Engine(pb, me, **FAST) as eng:
+            eng.set_tiles(True, 64, 16)
+            eng.set_tile_shapes(shapes)
+            V = U.copy()
+            vc = eng.ode_solve(V)[0]
+        np.testing.assert_array_equal(V, ref, err_msg=str(rows))
+        assert vc == out_ref[0]
+
+
 def test_kernel_timing_does_not_change_results():
     """Timed V-cycles are launched directly instead of from the graph: the
     run must be bit-identical, and the timed kernel must have been timed."""
