@@ -52,7 +52,7 @@ def _gpu_solve(pb, me, U, tiled, tile=(64, 16), **kw):
     return V, out
 
 
-@pytest.mark.parametrize("n,levels", [(65, 3), (129, 4), (257, 5)])
+@pytest.mark.parametrize("n,levels", [(65, 3), (129, 4), (257, 5), (129, 3)])
 @pytest.mark.parametrize("ncap", [1, 3])
 @pytest.mark.parametrize("n_coarse", [0, 32])
 def test_fast_fp64_tiled_equals_untiled_equals_oracle(n, levels, ncap, n_coarse):
