@@ -730,8 +730,8 @@ static TileShape tile_shape(const qdeq_ctx *c, int level, bool down) {
   return TileShape{c->tile_tx, c->tile_ty, TY_THREADS};
 }
 
-// Whether a shape fits the kernels: even sizes, the 5 x 64 columns of
-// tile_load, and the LDS with 8 KiB to spare.
+// Whether a shape fits the kernels: even sizes, tile rows (with halo) of at
+// most 320 points, and the LDS with 8 KiB to spare.
 static bool shape_fits(const qdeq_ctx *c, const TileShape &s, bool down,
                        bool has_f, size_t esz) {
   const int nu = down ? c->cfg.n_pre : c->cfg.n_post;
@@ -751,7 +751,7 @@ bool tiles_fit(const qdeq_ctx *c, int tx, int ty) {
                                      TILE_THREADS);
   const size_t up = tile_lds_bytes(tx, ty, 2 * c->cfg.n_post + 1, true, esz, false,
                                    TILE_THREADS);
-  // tile_load covers at most 5 x 64 columns per row
+  // tile rows (with halo) of at most 320 points
   if (tx + 2 * (2 * c->cfg.n_pre + 2) > 5 * TX_THREADS) return false;
   if (tx + 2 * (2 * c->cfg.n_post + 1) > 5 * TX_THREADS) return false;
   return down + 8192 <= c->lds_max && up + 8192 <= c->lds_max;
