@@ -13,8 +13,9 @@ bundled HIP runtime would replace the one the library was built against).
 With --rk-steps K it instead runs K accepted RK43 steps (after --rk-warmup
 untimed ones) of the basin workload at residual --tol, with the projection
 start from --proj-slots stored solutions (0: the previous solution), and
-prints the rate.  Under a kernel tracer run it with QDEQ_DIRECT_LAUNCH=1 so
-the V-cycles are launched kernel by kernel instead of as HIP graphs.
+prints the rate.  A single simulation launches its V-cycle kernels one by
+one already; QDEQ_DIRECT_LAUNCH=1 under a kernel tracer makes sure no HIP
+graph launch reaches it.
 """
 import argparse
 import json
