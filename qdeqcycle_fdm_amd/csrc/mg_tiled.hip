@@ -2,15 +2,18 @@
 //
 // One V-cycle of the fast cycle costs two launches per level above the coarse
 // tail plus one launch for the whole tail:
-//   k_down  (level l): nu red/black sweeps, residual, full-weighting
+//   k_down  (level l): n_pre red/black sweeps, residual, full-weighting
 //           restriction to level l+1, all on one LDS tile with a halo of
-//           2 nu + 2 points, so the ring of redundant updates makes every
+//           2 n_pre + 2 points, so the ring of redundant updates makes every
 //           interior value exactly what global sweeps would produce;
 //   k_tail  (levels with N <= TAIL_MAX_N): one workgroup runs the rest of the
 //           V-cycle in LDS with workgroup barriers only;
-//   k_up    (level l): bilinear prolongation + correction, nu sweeps, and on
-//           level 0 either the residual-norm partials (FP64 solve) or the
-//           FP64 update u += e (FP32 correction of iterative refinement).
+//   k_up    (level l): bilinear prolongation + correction (folded into the
+//           tile load), n_post sweeps, and on level 0 either the
+//           residual-norm partials (FP64 solve) or the FP64 update u += e
+//           (FP32 correction of iterative refinement).
+// Every loop over a tile window deals its points to all threads of the
+// workgroup in turn (FlatWalk).
 // Every update, residual, restriction and prolongation is evaluated with the
 // same expressions as the one-launch-per-half-sweep kernels of mg.hip, so the
 // tiled cycle reproduces them bit for bit.  The input and output u of a tile
