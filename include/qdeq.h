@@ -129,11 +129,13 @@ int qdeq_kernel_breakdown(qdeq_ctx *ctx, int reps, int *n, double *us);
 
 /* Measurement hooks (no reference counterpart).  qdeq_set_kernel_timing
  * brackets one kernel of the tiled fast cycle with HIP events on the
- * context's stream: every 8th V-cycle is launched directly instead of as
- * the captured graph (same kernels, same arguments) with the events recorded
- * right before and after that kernel; qdeq_get_kernel_timing reads the
- * accumulated duration and the count of timed launches.  qdeq_copy_probe times `reps` launches of a plain
- * device copy of n_doubles doubles (bandwidth and counter calibration). */
+ * context's stream: every 8th V-cycle is launched kernel by kernel (also
+ * where V-cycles are otherwise graph replays; same kernels, same arguments)
+ * with the events recorded right before and after that kernel;
+ * qdeq_get_kernel_timing reads the accumulated duration and the count of
+ * timed launches (a sampled V-cycle that the stop test skipped is not
+ * counted).  qdeq_copy_probe times `reps` launches of a plain device copy
+ * of n_doubles doubles (bandwidth and counter calibration). */
 enum { QDEQ_TIME_OFF = 0, QDEQ_TIME_UP0 = 1, QDEQ_TIME_TAIL = 2,
        QDEQ_TIME_DOWN1 = 3, QDEQ_TIME_FAULT = 4 };
 /* QDEQ_TIME_FAULT times the fault update after each stage solve (traction,
