@@ -2,35 +2,38 @@
 """Accepted RK43 steps per second at 1025^2 to a 1e-10 relative elliptic
 residual, on one MI355X.
 
-Workload (basin_1025): 48 km x 48 km antiplane fault model on a 1025 x 1025
-grid, low-rigidity basin (20 GPa in the top 2 km within 10 km of the fault,
-500 m tanh edges, 32 GPa elsewhere), rate-and-state friction with radiation
-damping, 7-level multigrid, 1e-10 relative residual per stage solve.  The
-default run warms up W steps from t = 0 and times the next K accepted steps
-(W + K = 3020: the workload's 3000-step run, which spans its first seismic
-events).  Inputs are synthetic (generated, no data files).
+Workload (basin_1025, qdeqcycle_fdm_amd/params/basin_1025.json): 48 km x
+48 km antiplane fault model on a 1025 x 1025 grid, low-rigidity basin (20 GPa
+in the top 2 km within 10 km of the fault, 500 m tanh edges, 32 GPa
+elsewhere), rate-and-state friction with radiation damping, 7-level
+multigrid, 1e-10 relative residual per stage solve.  The run warms up W steps
+from t = 0 and times the next K accepted steps (`value` = K / seconds, inputs
+in HBM).  Inputs are synthetic (generated, no data files).
 
 Modes: fast64 (the headline: FP64 V(1,3) fast cycle on the tiled LDS
 kernels, direct coarsest solve), ir32 (FP32 correction cycles in FP64
 iterative refinement) and ref64 (the reference's own V-cycle: 32 sweeps per
 level, FP64).  One JSON line goes to stdout; progress goes to stderr.
 
-Learned vs plain: after the plain fast64 run, pairs are harvested from
---harvest-runs runs of the same workload with seeded Dc / sigma_n
-perturbations, the network is trained on the GPU (both untimed, their times
-reported), and the fast64 run is repeated with the learned initial guess
-from the same start to the end of a common window: the first
---window-steps timed plain steps, which end in the quiet phase after the
-first event (the 3000-step run itself ends inside the second event, whose
-onset shifts by ~1e-5 of the elapsed time with any change of the solves'
-roundoff, so its end state is not comparable).  V-cycles per solve, steps
-per second, simulated seconds per second and the slip difference at the
-window's end are reported for both; `value` is the plain run's.
-
-The CPU baseline is the project's FP64 port of the solver (oracle/) on one
-host core, restarted from the GPU's state at the end of the timed region and
-run for about --cpu-seconds; the GPU then runs the same steps and the two
-final states are compared bit for bit.
+Side measurements, each of a fixed size that does not depend on K:
+* comparison window: the same warm-up, then --window-steps plain steps,
+  which end in the quiet phase after the first event (the headline run
+  itself when K reaches the window, else a separate run);
+* learned vs plain: pairs are harvested from --harvest-runs runs of
+  --harvest-steps steps with seeded Dc / sigma_n perturbations, the network
+  is trained on the GPU (both untimed, their times reported), and the run
+  with the learned initial guess goes from t = 0 to the window's end;
+  V-cycles per solve, steps per second and the slip difference at the
+  window's end are reported for both;
+* projection start over the same window, and the plain start at 1e-12;
+* ref64 over the headline's first min(K, --ref-steps) timed steps, against
+  fast64's rate over the same steps (speedup_vs_ref64), and ir32;
+* an ensemble leg: --ensemble-leg members of the seeded 64-member draw,
+  --ensemble-leg-steps steps each, alone and stepped together;
+* the CPU baseline: the project's FP64 port of the solver (oracle/) restarted
+  from the GPU's state at the end of the timed region, on one host core for
+  about --cpu-seconds, then the same steps on every core; the GPU then runs
+  the same steps and the three final states are compared bit for bit.
 """
 import argparse
 import json
@@ -54,7 +57,7 @@ MODES = {
     "ir32": dict(cycle=1, precision=1, n_pre=1, n_post=3, n_coarse=0,
                  relax=1.15, tiles=(64, 32)),
     "ref64": dict(cycle=0, precision=0, n_pre=32, n_post=32, n_coarse=32,
-                  tiles=None),
+                  relax=1.0, tiles=None),
 }
 
 HBM_PEAK_GBS = 8000.0
@@ -107,10 +110,12 @@ def kernel_bytes(mode, n0, nu):
     return per * s * n0
 
 
-def run_mode(pb, me, mode, warmup, steps, timed_kernel=None, split_at=None):
-    """Warm-up, then `steps` timed accepted steps.  With split_at, the state
-    after split_at timed steps is copied out between two timed segments (the
-    copy is not timed) and the first segment's figures are kept apart."""
+def run_mode(pb, me, mode, warmup, steps, timed_kernel=None, marks=(), keep_state_at=None):
+    """Warm-up, then `steps` timed accepted steps.  `marks`: step counts
+    inside the timed region at which the cumulative rate is recorded (the
+    timed region is split there, with a device barrier between segments);
+    with keep_state_at (one of the marks) the state there is copied out
+    between segments (the copy is not timed)."""
     from qdeqcycle_fdm_amd import lib
     eng = make_engine(pb, me, mode)
     eng.rk43_setup()
@@ -120,31 +125,31 @@ def run_mode(pb, me, mode, warmup, steps, timed_kernel=None, split_at=None):
     if timed_kernel is not None:
         eng.set_kernel_timing(timed_kernel)
     i0 = eng.state()["info"]
-    segs = [steps] if not split_at or split_at >= steps else [split_at, steps - split_at]
-    seconds, infos, split_state = 0.0, [], None
-    for n, k in enumerate(segs):
+    cuts = sorted({m for m in marks if 0 < m < steps} | {steps})
+    seconds, done, at, kept = 0.0, 0, {}, None
+    for cut in cuts:
         t0 = time.perf_counter()
-        info = eng.rk43_step(k)
+        info = eng.rk43_step(cut - done)
         sync()
         seconds += time.perf_counter() - t0
-        infos.append((info, seconds))
-        if n == 0 and len(segs) > 1:
-            split_state = eng.state()
+        done = cut
+        at[cut] = dict(steps=cut, seconds=seconds, steps_per_s=cut / seconds,
+                       t_start=i0["t"], t_end=info["t"],
+                       vcycles_per_solve=(info["vcycles"] - i0["vcycles"]) /
+                       max(info["solves"] - i0["solves"], 1),
+                       solves_per_step=(info["solves"] - i0["solves"]) / cut,
+                       sim_s_per_s=(info["t"] - i0["t"]) / seconds)
+        if cut == keep_state_at and cut < steps:
+            kept = eng.state()
     out = dict(seconds=seconds, steps=steps, steps_per_s=steps / seconds,
                vcycles=info["vcycles"] - i0["vcycles"],
                solves=info["solves"] - i0["solves"], t_start=i0["t"],
-               t_end=info["t"], V_max_end=info["V_max"])
+               t_end=info["t"], V_max_end=info["V_max"], marks=at)
     out["vcycles_per_solve"] = out["vcycles"] / max(out["solves"], 1)
     out["solves_per_step"] = out["solves"] / steps
-    if split_state is not None:
-        f, fs = infos[0]
-        out["window"] = dict(steps=segs[0], seconds=fs, steps_per_s=segs[0] / fs,
-                             t_start=i0["t"], t_end=f["t"],
-                             vcycles_per_solve=(f["vcycles"] - i0["vcycles"]) /
-                             max(f["solves"] - i0["solves"], 1),
-                             solves_per_step=(f["solves"] - i0["solves"]) / segs[0],
-                             sim_s_per_s=(f["t"] - i0["t"]) / fs)
-        out["window_state"] = split_state
+    if keep_state_at == steps:
+        kept = eng.state()
+    out["kept_state"] = kept
     if timed_kernel is not None:
         ms, calls = eng.kernel_timing()
         out["kernel_ms_total"], out["kernel_calls"] = ms, calls
@@ -440,40 +445,67 @@ def run_mixed_2049(n_events=2, tail_steps=200):
                 ir32_2049_vs_fp64_1025=catalogue(ir, ref))
 
 
+def cpu_cores() -> int:
+    """Host cores this process may use (the affinity mask, capped by
+    OMP_NUM_THREADS when that is set)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return min(n, int(env)) if env.isdigit() and int(env) > 0 else n
+
+
 def cpu_baseline(pb, me, eng, seconds):
-    """Oracle restarted from the engine's state; the engine then runs the
-    same steps; returns the CPU rate, the GPU rate on the same steps and
-    whether the final states agree bit for bit."""
+    """The oracle restarted from the engine's state: on one thread for about
+    `seconds`, then the same steps on every core (OpenMP over the
+    multigrid's point loops); the engine then runs the same steps.  Returns
+    both CPU rates, the GPU rate on the same steps and whether the three
+    final states agree bit for bit."""
     import numpy as np
     import oracle
     st = eng.state()
     info = st["info"]
     p = MODES["fast64"]
-    sim = oracle.Sim(pb, me, cycle=1, n_pre=p["n_pre"], n_post=p["n_post"],
-                     n_coarse=p["n_coarse"], relax=p["relax"])
     oracle.set_libm(False)
-    sim.set_state(info["t"], info["dt"], info["it"], info["errA0"], info["errA1"],
-                  st["U"], st["slip"], st["psi"])
+
+    def restart():
+        sim = oracle.Sim(pb, me, cycle=1, n_pre=p["n_pre"], n_post=p["n_post"],
+                         n_coarse=p["n_coarse"], relax=p["relax"])
+        sim.set_state(info["t"], info["dt"], info["it"], info["errA0"], info["errA1"],
+                      st["U"], st["slip"], st["psi"])
+        return sim
+
+    oracle.set_threads(1)
+    sim1 = restart()
     n = 0
     t0 = time.perf_counter()
     while True:
-        sim.step()
+        sim1.step()
         n += 1
         if time.perf_counter() - t0 >= seconds:
             break
-    cpu_s = time.perf_counter() - t0
+    cpu1_s = time.perf_counter() - t0
+    cores = cpu_cores()
+    used = oracle.set_threads(cores)
+    simn = restart()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        simn.step()
+    cpun_s = time.perf_counter() - t0
+    oracle.set_threads(1)
     sync()
     g0 = time.perf_counter()
     eng.rk43_step(n)
     sync()
     gpu_s = time.perf_counter() - g0
     st2 = eng.state()
-    ar = sim.arrays()
-    same = all(np.array_equal(st2[k], ar[k]) for k in ("slip", "psi", "V", "U"))
-    same = same and st2["info"]["t"] == sim.scalars()["t"]
-    return dict(steps=n, cpu_seconds=cpu_s, cpu_steps_per_s=n / cpu_s,
+    same = True
+    for sim in (sim1, simn):
+        ar = sim.arrays()
+        same = same and all(np.array_equal(st2[k], ar[k]) for k in ("slip", "psi", "V", "U"))
+        same = same and st2["info"]["t"] == sim.scalars()["t"]
+    return dict(steps=n, cpu_seconds=cpu1_s, cpu_steps_per_s=n / cpu1_s,
+                cpu_all_seconds=cpun_s, cpu_all_steps_per_s=n / cpun_s, threads=used,
                 gpu_steps_per_s_same_steps=n / gpu_s, bitwise_equal=bool(same),
-                t_start=info["t"])
+                t_start=info["t"], t_end=st2["info"]["t"])
 
 
 def main():
@@ -486,7 +518,9 @@ def main():
                     help="timed steps of the ir32 mode (ref64 runs a tenth)")
     ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="one-core CPU baseline length (the all-core run repeats "
+                         "the same steps)")
     ap.add_argument("--ensemble", type=int, default=0, metavar="M",
                     help="only run the ensemble workload: M members, "
                          "--ensemble-steps steps each, alone and together")
@@ -507,7 +541,17 @@ def main():
                     help="skip the learned-guess harvest, training and run")
     ap.add_argument("--harvest-runs", type=int, default=2)
     ap.add_argument("--window-steps", type=int, default=2780,
-                    help="timed plain steps of the learned-vs-plain window")
+                    help="plain steps of the learned-vs-plain window (after the "
+                         "warm-up; independent of --steps)")
+    ap.add_argument("--harvest-steps", type=int, default=3000,
+                    help="steps of each harvest run of the learned guess")
+    ap.add_argument("--ref-steps", type=int, default=300,
+                    help="at most this many of the headline's timed steps are "
+                         "also run with the reference cycle (speedup_vs_ref64)")
+    ap.add_argument("--ensemble-leg", type=int, default=8, metavar="M",
+                    help="members of the short ensemble leg of the default run "
+                         "(0: none)")
+    ap.add_argument("--ensemble-leg-steps", type=int, default=50)
     ap.add_argument("--train-epochs", type=int, default=30)
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="run the V-cycle as one captured graph without the "
@@ -552,13 +596,12 @@ def main():
                           "unit": "steps/s", "n_gpus": 1, "higher_is_better": True,
                           "data": "synthetic", "ensemble": res}), flush=True)
         return
-    pb = config.make_bp1_problem(1025, basin=True, tol_mode=config.TOL_REL,
-                                 tol=1e-10)
+    pb = config.workload("basin_1025")
     if args.copy_probe:
         with lib.Engine(pb) as eng:
             ms = eng.copy_probe(args.copy_probe, reps=10)
         print(json.dumps({"copy_probe_doubles": args.copy_probe, "avg_ms": ms,
-                          "GB/s": 2 * 8 * args.copy_probe / ms / 1e6}))
+                          "GB/s": 2 * 8 * args.copy_probe / ms / 1e9 * 1e3}))
         return
     t0 = time.perf_counter()
     me = lib.build_mu_equi(pb.mu)
@@ -566,28 +609,49 @@ def main():
 
     timed = (lib.TIME_UP0 if args.mode != "ref64" and not args.no_kernel_timing
              else None)
-    # The learned-vs-plain window ends in the quiet phase after the first
-    # event (step 2800 of the default run), where both runs can be stopped
-    # at the same time; the plain run's state there is copied out untimed.
-    split = (args.window_steps if args.mode == "fast64" and
-             not (args.no_learned and args.no_projection) else None)
+    # the reference cycle is timed over the headline's first ref_steps steps
+    # (its rate is 1/50 of the fast cycle's), and the fast cycle's rate over
+    # the same steps is recorded for the comparison
+    ref_steps = min(args.steps, args.ref_steps)
+    window = args.window_steps
+    compare = args.mode == "fast64" and not (args.no_learned and args.no_projection)
+    marks = [ref_steps] + ([window] if compare and window < args.steps else [])
     eng, main_run = run_mode(pb, me, args.mode, args.warmup, args.steps, timed,
-                             split_at=split)
+                             marks=marks,
+                             keep_state_at=window if compare and window <= args.steps
+                             else None)
     log(f"{args.mode}: {main_run['steps_per_s']:.2f} steps/s, "
         f"{main_run['vcycles_per_solve']:.2f} V-cycles/solve, "
         f"t {main_run['t_start']:.4g} -> {main_run['t_end']:.4g} s")
 
+    # The learned-vs-plain comparison window: the same warm-up, then
+    # window_steps plain steps, which end in the quiet phase after the first
+    # event (the state there is copied out untimed).  It does not depend on
+    # --steps: shorter headline runs get a separate plain run of the window.
+    win, win_state = None, None
+    if compare:
+        if window <= args.steps:
+            win, win_state = main_run["marks"][window], main_run["kept_state"]
+        else:
+            e_w, wr = run_mode(pb, me, "fast64", args.warmup, window, None,
+                               keep_state_at=window)
+            e_w.close()
+            win, win_state = wr["marks"][window], wr["kept_state"]
+        log(f"comparison window: {win['steps']} plain steps, t {win['t_start']:.4g} -> "
+            f"{win['t_end']:.4g} s, {win['steps_per_s']:.2f} steps/s, "
+            f"{win['vcycles_per_solve']:.3f} V-cycles/solve")
+    main_run.pop("kept_state", None)
+
     learned = None
-    if not args.no_learned and args.mode == "fast64" and "window" in main_run:
+    if compare and not args.no_learned:
         import numpy as np
-        g, tinfo = train_guess(me, args.warmup + args.steps, args.harvest_runs,
-                               2024, args.train_epochs)
-        win = main_run["window"]
+        g, tinfo = train_guess(me, args.harvest_steps, args.harvest_runs, 2024,
+                               args.train_epochs)
         e3, lr = run_learned(pb, me, g, args.warmup, win["t_end"])
         st3 = e3.state()
         e3.close()
         g.close()
-        ps = main_run["window_state"]["slip"]
+        ps = win_state["slip"]
         dslip = float(np.max(np.abs(st3["slip"] - ps)) / np.max(np.abs(ps)))
         log(f"window t {win['t_start']:.4g} -> {win['t_end']:.4g} s: plain "
             f"{win['steps']} steps, {win['steps_per_s']:.2f} steps/s, "
@@ -608,11 +672,10 @@ def main():
     # Projection start against the plain start over the same window, and the
     # accuracy of both against the plain start at a 100x tighter residual
     projection = None
-    if not args.no_projection and args.mode == "fast64" and "window" in main_run:
-        win = main_run["window"]
+    if compare and not args.no_projection:
         t_end = win["t_end"]
         ref, st_ref = run_projection(me, args.warmup, t_end, 0, 1e-12)
-        ds, dv = fault_diff(main_run["window_state"], st_ref)
+        ds, dv = fault_diff(win_state, st_ref)
         log(f"plain 1e-12 over the window: {ref['steps_per_s']:.2f} steps/s, "
             f"{ref['vcycles_per_solve']:.3f} V-cycles/solve; plain 1e-10 vs it: "
             f"slip {ds:.2e}, log10 V {dv:.2e}")
@@ -641,7 +704,6 @@ def main():
                 f"{pr['vcycles_per_solve']:.3f} V-cycles/solve, used "
                 f"{pr.get('used')}/{pr.get('offered')}; vs plain 1e-12: slip {ds:.2e}, "
                 f"log10 V {dv:.2e}")
-    main_run.pop("window_state", None)
 
     # per-kernel durations of the tiled cycle (each kernel launched back to
     # back on the engine's buffers, its shape as tuned)
@@ -687,7 +749,7 @@ def main():
 
     modes = {}
     if not args.no_compare:
-        for m, k in (("ir32", args.compare_steps), ("ref64", max(args.compare_steps // 10, 5))):
+        for m, k in (("ir32", args.compare_steps), ("ref64", ref_steps)):
             if m == args.mode:
                 continue
             e2, r = run_mode(pb, me, m, args.warmup, k)
@@ -698,23 +760,38 @@ def main():
             log(f"{m}: {r['steps_per_s']:.2f} steps/s, {r['vcycles_per_solve']:.2f} "
                 f"V-cycles/solve over {k} steps")
 
+    ensemble = None
+    if args.ensemble_leg and args.mode != "ref64":
+        ensemble = run_ensemble(args.ensemble_leg, args.ensemble_leg_steps,
+                                args.ensemble_mode, 0)
+        log(f"ensemble leg: {args.ensemble_leg} members x {args.ensemble_leg_steps} steps: "
+            f"{ensemble['steps_per_s_ensemble']:.1f} steps/s together, "
+            f"{ensemble['steps_per_s_alone']:.1f} one after the other, bitwise equal: "
+            f"{ensemble['bitwise_equal_to_alone']}")
+
     cpu = None
     if not args.no_cpu and args.mode == "fast64":
         cb = cpu_baseline(pb, me, eng, args.cpu_seconds)
-        log(f"cpu baseline: {cb['cpu_steps_per_s']:.3f} steps/s over {cb['steps']} "
-            f"steps; GPU on the same steps {cb['gpu_steps_per_s_same_steps']:.1f}; "
-            f"bitwise equal: {cb['bitwise_equal']}")
+        log(f"cpu baseline: {cb['cpu_steps_per_s']:.3f} steps/s on one core, "
+            f"{cb['cpu_all_steps_per_s']:.3f} on {cb['threads']} threads, over "
+            f"{cb['steps']} steps; GPU on the same steps "
+            f"{cb['gpu_steps_per_s_same_steps']:.1f}; bitwise equal: {cb['bitwise_equal']}")
         cpu = {"value": round(cb["cpu_steps_per_s"], 4), "unit": "steps/s",
                "cores": 1, "kind": "port",
                "sample": (f"{cb['steps']} accepted steps of basin_1025 restarted "
                           f"from the GPU state at t={cb['t_start']:.6g} s "
                           f"(after the timed region), same fast FP64 cycle, "
                           f"oracle/ C port on one core"),
+               "all_cores": {"value": round(cb["cpu_all_steps_per_s"], 4),
+                             "cores": cb["threads"],
+                             "sample": "the same steps, OpenMP over the multigrid's "
+                                       "point loops"},
                "gpu_steps_per_s_same_steps": round(cb["gpu_steps_per_s_same_steps"], 2),
                "bitwise_equal_to_gpu": cb["bitwise_equal"]}
     eng.close()
 
     head = main_run
+    same = main_run["marks"][ref_steps]
     line = {
         "metric": METRIC, "value": round(head["steps_per_s"], 3),
         "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
@@ -727,18 +804,27 @@ def main():
                    "residual": "1e-10 relative", "mode": args.mode,
                    "cycle": ("reference V(32,32)" if args.mode == "ref64" else
                              "V(1,3), red/black relaxation 1.15, direct 17x17 "
-                             "coarsest solve"),
+                             "coarsest solve (its inverse stored in FP32, "
+                             "products and sums in FP64)"),
                    "initial_guess": "previous solution",
                    "tile_shapes": TUNED.get(args.mode),
-                   "sim_time_s": [head["t_start"], head["t_end"]]},
+                   "timed_window": {"t_start_s": head["t_start"], "t_end_s": head["t_end"],
+                                    "after_warmup_steps": args.warmup,
+                                    "vcycles_per_solve": round(head["vcycles_per_solve"], 3),
+                                    "solves_per_step": round(head["solves_per_step"], 3)}},
         "vcycles_per_solve": round(head["vcycles_per_solve"], 3),
         "solves_per_step": round(head["solves_per_step"], 3),
         "vcycles_per_solve_learned_vs_plain": (
             None if learned is None else
             [round(learned["vcycles_per_solve"], 3),
              round(learned["plain_same_window"]["vcycles_per_solve"], 3)]),
+        "steps_per_s_learned_vs_plain": (
+            None if learned is None else
+            [round(learned["steps_per_s"], 2),
+             round(learned["plain_same_window"]["steps_per_s"], 2)]),
         "plain": {k: main_run[k] for k in ("steps", "steps_per_s",
                                             "vcycles_per_solve", "solves_per_step")},
+        "comparison_window": win,
         "learned": learned,
         "vcycles_per_solve_projection": (
             None if projection is None else
@@ -748,10 +834,14 @@ def main():
         "kernel_breakdown_us": breakdown,
         "cpu_baseline": cpu,
         "modes": modes,
+        "ensemble": ensemble,
     }
     if "ref64" in modes:
-        line["speedup_vs_ref64"] = round(head["steps_per_s"] /
-                                         modes["ref64"]["steps_per_s"], 2)
+        line["speedup_vs_ref64"] = {
+            "steps": ref_steps, "fast64_steps_per_s": round(same["steps_per_s"], 2),
+            "ref64_steps_per_s": round(modes["ref64"]["steps_per_s"], 2),
+            "speedup": round(same["steps_per_s"] / modes["ref64"]["steps_per_s"], 2),
+            "same_t_end": modes["ref64"]["t_end"] == same["t_end"]}
     print(json.dumps(line), flush=True)
 
 

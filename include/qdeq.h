@@ -21,6 +21,9 @@
  *   - every entry point returns a qdeq_status instead of calling exit()
  *     (helper_cuda.h:540-555, fault.cpp:218);
  *   - array arguments carry a pointer-space flag (QDEQ_HOST / QDEQ_DEVICE);
+ *     every array of one call lives in that space; device arrays must be on
+ *     the context's device, and work the caller queued on the null stream
+ *     (stream 0) is complete before the call reads them;
  *   - all solver state lives in a context (the reference keeps it in
  *     file-scope globals, odeSolver.cpp:6-12), so several contexts may coexist;
  *     one context must not be used from two host threads at once.
@@ -150,6 +153,12 @@ int qdeq_copy_probe(qdeq_ctx *ctx, long long n_doubles, int reps, double *avg_ms
 const char *qdeq_last_error(const qdeq_ctx *ctx);
 /* Message of the last failing qdeq_create / context-free call. */
 const char *qdeq_global_error(void);
+/* HIP runtime version (HIP_VERSION encoding: major*1e7 + minor*1e5 + patch)
+ * the library was built against, and the one the process runs (no
+ * reference counterpart).  They differ when another library loaded first
+ * brought its own HIP runtime under the same soname; callers should refuse
+ * to run then.  running may be NULL. */
+int qdeq_hip_runtime_version(int *built, int *running);
 
 /* Writes the doubled equivalent-medium array the context uses (built on the
  * device when qdeq_create got NULL).  where: QDEQ_HOST / QDEQ_DEVICE. */

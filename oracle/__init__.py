@@ -13,7 +13,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liboracle_qdeq.so")
+# QDEQ_ORACLE_LIB selects another build of the same source (the RK43 checks
+# load builds with planted mistakes this way, in a subprocess)
+LIB_PATH = os.environ.get("QDEQ_ORACLE_LIB") or os.path.join(_HERE, "liboracle_qdeq.so")
 
 ORC_TOL_ABS, ORC_TOL_REL = 0, 1
 ORC_OK, ORC_ERR_ARG, ORC_ERR_NAN_FRICTION, ORC_ERR_MG_NOT_CONVERGED = 0, 1, 2, 3
@@ -119,6 +121,14 @@ def lib():
         L.orc_sim_step.argtypes = [ctypes.c_void_p]
         L.orc_sim_step.restype = ctypes.c_int
         L.orc_sim_scalars.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_set_threads.restype = ctypes.c_int
+        L.orc_sim_set_dt.argtypes = [ctypes.c_void_p, ctypes.c_double]
+        L.orc_sim_set_dt.restype = None
+        L.orc_sim_last_attempt.argtypes = [ctypes.c_void_p] + [_dp] * 8
+        L.orc_sim_last_attempt.restype = None
+        L.orc_sim_stage_log.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_sim_stage_log.restype = ctypes.c_int
         L.orc_sim_arrays.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp]
         _lib = L
     return _lib
@@ -145,6 +155,12 @@ def set_libm(use_glibc: bool) -> None:
     """Selects glibc (True) or the engine's algorithms (False) for the
     exp/sinh/cosh/asinh calls of NRsearch and cal_dpsi."""
     lib().orc_set_libm(1 if use_glibc else 0)
+
+
+def set_threads(n: int) -> int:
+    """OpenMP threads of the oracle's multigrid loops (results do not depend
+    on it); returns the count in use."""
+    return lib().orc_set_threads(int(n))
 
 
 def tableau() -> dict:
@@ -318,6 +334,31 @@ class Sim:
         st = lib().orc_sim_step(self.h)
         if st != ORC_OK:
             raise RuntimeError(f"oracle step failed: status {st}")
+
+    def set_dt(self, dt: float):
+        """Step size of the next attempt (test hook)."""
+        lib().orc_sim_set_dt(self.h, float(dt))
+
+    def last_attempt(self) -> dict:
+        """Stage rates V, dpsi and stage states slip, psi of stages 2..6 of
+        the last attempt (rows 2..6 of (7, nz) arrays; rows 0-1 zero) and its
+        y3 / y4 solutions."""
+        nz = self.nz
+        bufs = {k: np.zeros((6, nz)) for k in ("V", "dpsi", "slip", "psi")}
+        ys = {k: np.zeros(nz) for k in ("y3s", "y3p", "y4s", "y4p")}
+        lib().orc_sim_last_attempt(self.h, _p(bufs["V"]), _p(bufs["dpsi"]),
+                                   _p(bufs["slip"]), _p(bufs["psi"]),
+                                   *[_p(ys[k]) for k in ("y3s", "y3p", "y4s", "y4p")])
+        out = {k: np.vstack([np.zeros((1, nz)), v]) for k, v in bufs.items()}
+        out.update(ys)
+        return out
+
+    def stage_log(self) -> np.ndarray:
+        """(n, 5): t, dt, ratio, U at row 1 of the remote and of the fault
+        column right after setBoundary, for the last n <= 8 stage calls."""
+        o = np.zeros(40)
+        n = lib().orc_sim_stage_log(self.h, _p(o))
+        return o[:5 * n].reshape(n, 5)
 
     def scalars(self) -> dict:
         o = (ctypes.c_double * 11)()

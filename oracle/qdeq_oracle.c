@@ -204,6 +204,26 @@ void orc_mu_equi(int ni, int nk, const double *mu, double *me) {
 /* Multigrid (odeSolver.cpp:652-1147)                                        */
 /* ------------------------------------------------------------------------ */
 
+/* Threads: the loops over the points of one colour, of a residual, a
+ * restriction or a prolongation update independent points, so they are
+ * split over OpenMP threads by rows without changing any result; sums (the
+ * residual norm) stay sequential.  Grids below ORC_OMP_MIN points run on
+ * one thread. */
+#define ORC_OMP_MIN 16384
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
 /* Level hierarchy, odeSolver.cpp:1384-1405: N=nk, K=ni at level 0,
  * N <- (N-1)/2+1, dh doubled, fin_coa = 2^level. */
 void orc_level_dims(int ni, int nk, double dh, int n_levels, int *N, int *K,
@@ -251,6 +271,7 @@ void orc_rbgs(double *u, const double *f, const double *mu, int ni, int nk,
   orc_level_dims(ni, nk, dh, level + 1, N, K, FC, DH);
   const long Nn = N[level], Kk = K[level], W2 = 2L * (ni + 2) - 1;
   const double DH2 = DH[level] * DH[level];
+#pragma omp parallel for schedule(static) if (Nn * Kk > ORC_OMP_MIN)
   for (long j = 0; j < Nn; j++)
     for (long k = 1; k < Kk - 1; k++) {
       if ((0 + j + k) % 2 != phase) continue;
@@ -290,6 +311,7 @@ void orc_residual(double *r, const double *u, const double *f,
   orc_level_dims(ni, nk, dh, level + 1, N, K, FC, DH);
   const long Nn = N[level], Kk = K[level], W2 = 2L * (ni + 2) - 1;
   const double DH2 = DH[level] * DH[level];
+#pragma omp parallel for schedule(static) if (Nn * Kk > ORC_OMP_MIN)
   for (long j = 0; j < Nn; j++)
     for (long k = 1; k < Kk - 1; k++) {
       long i1, i2, i3, i4;
@@ -328,6 +350,7 @@ void orc_restrict(double *r2, const double *r, double *e2, int ni, int nk,
   orc_level_dims(ni, nk, 1.0, level + 1, N, K, FC, DH);
   const long Nc = N[level], Kc = K[level], Kf = K[level - 1];
 #define RF(jj, kk) r[(long)(kk) + (long)(jj) * Kf]
+#pragma omp parallel for schedule(static) if (Nc * Kc > ORC_OMP_MIN)
   for (long j = 0; j < Nc; j++)
     for (long k = 1; k < Kc - 1; k++) {
       const long c = k + j * Kc;
@@ -371,6 +394,7 @@ void orc_prolong_full(double *u, const double *e2, int ni, int nk, int level,
   const long Nc = N[level], Kc = K[level], Kf = K[level - 1];
 #define UF(jj, kk) u[(long)(kk) + (long)(jj) * Kf]
 #define EC(jj, kk) e2[(long)(kk) + (long)(jj) * Kc]
+#pragma omp parallel for schedule(static) if (Nc * Kc > ORC_OMP_MIN)
   for (long j = 0; j < Nc - 1; j++)
     for (long k = 0; k < Kc - 1; k++) {
       UF(2 * j, 2 * k) += EC(j, k);
@@ -531,6 +555,7 @@ static void orc_coarse_direct(orc_mg *mg, double *u, const double *f,
    * summed in column order from 0, the chunk sums added in chunk order */
   const long Q = (1024 / n) < 4 ? (1024 / n < 1 ? 1 : 1024 / n) : 4;
   const long C = (n + Q - 1) / Q;
+#pragma omp parallel for schedule(static) if (n > 256)
   for (long row = 0; row < n; row++) {
     double acc = 0.0;
     for (long q = 0; q < Q; q++) {
@@ -616,6 +641,7 @@ int orc_ode_solve(orc_mg *mg, double *U, int tol_mode, double tol,
   const int ni = mg->ni, nk = mg->nk;
   const long nx = ni + 2, K = ni;
   const long n0 = (long)ni * nk;
+#pragma omp parallel for schedule(static) if (n0 > ORC_OMP_MIN)
   for (long j = 1; j <= nk; j++)
     for (long i = 1; i <= ni; i++) mg->u[(i - 1) + (j - 1) * K] = U[i + j * nx];
 
@@ -646,6 +672,7 @@ int orc_ode_solve(orc_mg *mg, double *U, int tol_mode, double tol,
     delta = orc_nrm2(mg->delta, n0);
     it++;
   }
+#pragma omp parallel for schedule(static) if (n0 > ORC_OMP_MIN)
   for (long j = 1; j <= nk; j++)
     for (long i = 1; i <= ni; i++) U[i + j * nx] = mg->u[(i - 1) + (j - 1) * K];
   orc_extend_crew(U, ni, nk);
@@ -951,7 +978,20 @@ struct orc_sim {
   int it, attempts;
   long vcycles, solves;
   orc_mg *mg;
+  /* ring of the last 8 stage calls (t, dt, ratio, remote and fault column
+   * values of row 1 right after setBoundary); log_n calls so far */
+  double log[8][5];
+  long log_n;
 };
+
+/* Planted mistakes for the tests that show the RK43 checks detect them: the
+ * tests build this file with -DORC_MUTANT=n into a scratch directory.
+ * 1 swaps a42 and a43 in stage 4's sums, 2 hands stage 4 the boundary time
+ * of stage 3, 3 commits the embedded third-order solution.  0 (the default)
+ * is the restatement itself. */
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
 
 /* Butcher tableau, odeSolver.cpp:1297-1334. */
 static const double c2 = 1. / 2., c3 = 83. / 250., c4 = 31. / 50.,
@@ -1070,6 +1110,15 @@ static int orc_stage(orc_sim *s, const double *ys, const double *yp,
   int vc = 0, st;
   orc_set_boundary(s->U, p->ni, p->nk, ys, s->slip_shift, p->Vini, s->t,
                    ratio, s->dt);
+  {
+    double *r = s->log[s->log_n % 8];
+    r[0] = s->t;
+    r[1] = s->dt;
+    r[2] = ratio;
+    r[3] = s->U[1 * s->nx + p->ni];
+    r[4] = s->U[1 * s->nx + 1];
+    s->log_n++;
+  }
   st = orc_ode_solve(s->mg, s->U, p->tol_mode, p->tol, p->max_vcycles, &vc,
                      NULL, NULL);
   s->vcycles += vc;
@@ -1147,13 +1196,15 @@ int orc_sim_step(orc_sim *s) {
     double *k4s = s->ks[4], *k4p = s->kp[4];
     for (long k = 1; k <= nk; k++) {
       k4s[k] = s->slip[k] + a41 * dt * fV[1][k];
-      k4s[k] = k4s[k] + a42 * dt * fV[2][k];
-      k4s[k] = k4s[k] + a43 * dt * fV[3][k];
+      k4s[k] = k4s[k] + (ORC_MUTANT == 1 ? a43 : a42) * dt * fV[2][k];
+      k4s[k] = k4s[k] + (ORC_MUTANT == 1 ? a42 : a43) * dt * fV[3][k];
       k4p[k] = s->psi[k] + a41 * dt * fd[1][k];
-      k4p[k] = k4p[k] + a42 * dt * fd[2][k];
-      k4p[k] = k4p[k] + a43 * dt * fd[3][k];
+      k4p[k] = k4p[k] + (ORC_MUTANT == 1 ? a43 : a42) * dt * fd[2][k];
+      k4p[k] = k4p[k] + (ORC_MUTANT == 1 ? a42 : a43) * dt * fd[3][k];
     }
-    if ((st = orc_stage(s, k4s, k4p, c4, fV[4], fd[4])) != ORC_OK) return st;
+    if ((st = orc_stage(s, k4s, k4p, ORC_MUTANT == 2 ? c3 : c4, fV[4], fd[4])) !=
+        ORC_OK)
+      return st;
     double *k5s = s->ks[5], *k5p = s->kp[5];
     for (long k = 1; k <= nk; k++) {
       k5s[k] = s->slip[k] + a51 * dt * fV[1][k];
@@ -1211,13 +1262,17 @@ int orc_sim_step(orc_sim *s) {
   }
   s->attempts = attemptCount;
   s->t = s->t + s->dt; /* 1605-1614 */
-  for (long k = 1; k <= nk; k++) {
-    s->slip[k] = s->y4s[k];
-    s->ks[1][k] = s->y4s[k];
-    fV[1][k] = 0.0;
-    s->psi[k] = s->y4p[k];
-    s->kp[1][k] = s->y4p[k];
-    fd[1][k] = 0.0;
+  {
+    const double *ns = ORC_MUTANT == 3 ? s->y3s : s->y4s;
+    const double *np = ORC_MUTANT == 3 ? s->y3p : s->y4p;
+    for (long k = 1; k <= nk; k++) {
+      s->slip[k] = ns[k];
+      s->ks[1][k] = ns[k];
+      fV[1][k] = 0.0;
+      s->psi[k] = np[k];
+      s->kp[1][k] = np[k];
+      fd[1][k] = 0.0;
+    }
   }
   if ((st = orc_stage(s, s->ks[1], s->kp[1], 0.0, fV[1], fd[1])) != ORC_OK)
     return st;
@@ -1231,6 +1286,42 @@ int orc_sim_step(orc_sim *s) {
   s->errA[0] = s->totErr;
   s->dt = newDeltaT;
   return ORC_OK;
+}
+
+void orc_sim_set_dt(orc_sim *s, double dt) { s->dt = dt; }
+
+/* Stage data of the last attempt of the last step: the stage rates fV, fd
+ * and states ks, kp of stages 2..6 (nz each, at offset (i-1)*nz for stage i;
+ * stage 1's entries are zero here, its values being those before the step)
+ * and the embedded (y3) and propagated (y4) solutions. */
+void orc_sim_last_attempt(const orc_sim *s, double *fV, double *fd, double *ks,
+                          double *kp, double *y3s, double *y3p, double *y4s,
+                          double *y4p) {
+  const size_t nz = (size_t)s->nz, nb = nz * sizeof(double);
+  for (int i = 1; i <= 6; i++) {
+    const size_t o = (size_t)(i - 1) * nz;
+    if (i == 1) {
+      memset(fV + o, 0, nb); memset(fd + o, 0, nb);
+      memset(ks + o, 0, nb); memset(kp + o, 0, nb);
+      continue;
+    }
+    memcpy(fV + o, s->fV[i], nb); memcpy(fd + o, s->fd[i], nb);
+    memcpy(ks + o, s->ks[i], nb); memcpy(kp + o, s->kp[i], nb);
+  }
+  memcpy(y3s, s->y3s, nb); memcpy(y3p, s->y3p, nb);
+  memcpy(y4s, s->y4s, nb); memcpy(y4p, s->y4p, nb);
+}
+
+/* The last min(8, calls) stage calls, oldest first, 5 doubles each:
+ * t, dt, ratio, U(row 1, remote column), U(row 1, fault column) right after
+ * setBoundary.  Returns the number of records. */
+int orc_sim_stage_log(const orc_sim *s, double *out) {
+  const long n = s->log_n < 8 ? s->log_n : 8;
+  for (long q = 0; q < n; q++) {
+    const double *r = s->log[(s->log_n - n + q) % 8];
+    for (int i = 0; i < 5; i++) out[5 * q + i] = r[i];
+  }
+  return (int)n;
 }
 
 /* maxval, fault.cpp:22-52 */

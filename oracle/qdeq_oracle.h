@@ -59,6 +59,10 @@ typedef struct orc_params {
 /* ---- medium (odeSolver.cpp:337-589) ---- */
 void orc_mu_equi(int ni, int nk, const double *mu, double *mu_equi);
 
+/* OpenMP threads of the multigrid loops (n > 0 sets; returns the count in
+ * use; 1 in a build without OpenMP).  Results do not depend on it. */
+int orc_set_threads(int n);
+
 /* ---- multigrid building blocks (odeSolver.cpp:741-1147) ---- */
 void orc_level_dims(int ni, int nk, double dh, int n_levels, int *N, int *K,
                     int *fin_coa, double *DH);
@@ -127,6 +131,16 @@ int orc_sim_set_state(orc_sim *s, double t, double dt, int it, double errA0,
                       double errA1, const double *U, const double *slip,
                       const double *psi);
 int orc_sim_step(orc_sim *s);          /* one accepted step, 1462-1633 */
+/* Test hooks of the RK43 checks: the step size of the next attempt, the
+ * last attempt's stage rates / states (stages 2..6 at offset (i-1)*nz) and
+ * y3 / y4, and the log of the last stage calls (t, dt, ratio, U at row 1 of
+ * the remote and of the fault column after setBoundary; 5 doubles each,
+ * oldest first, at most 8; returns the count). */
+void orc_sim_set_dt(orc_sim *s, double dt);
+void orc_sim_last_attempt(const orc_sim *s, double *fV, double *fd, double *ks,
+                          double *kp, double *y3s, double *y3p, double *y4s,
+                          double *y4p);
+int orc_sim_stage_log(const orc_sim *s, double *out);
 /* scalars: t, dt, it, errA0, errA1, totErr, attempts, vcycles(total),
  * solves(total), V_max, Ts_max */
 void orc_sim_scalars(const orc_sim *s, double *out11);

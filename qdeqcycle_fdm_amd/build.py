@@ -3,6 +3,9 @@
 Every HIP source is compiled with -ffp-contract=off (so FP64 expressions are
 evaluated in the order written, as the reference writes them) and with
 packed FP32 instructions disabled.  The library lands next to this file.
+
+Also builds _build/device_path, a C program of the test suite that calls the
+drop-in entry points with host and with device arrays (tests/c).
 """
 from __future__ import annotations
 
@@ -20,6 +23,9 @@ SOURCES = ["mg.hip", "mg_tiled.hip", "coarse.hip", "fault.hip", "medium.hip", "a
 ARCH = os.environ.get("QDEQ_OFFLOAD_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+DEVICE_PATH_SRC = os.path.join(ROOT, "tests", "c", "device_path.c")
+DEVICE_PATH_BIN = os.path.join(BUILD_DIR, "device_path")
 COMMON_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     "-ffp-contract=off",
@@ -61,7 +67,20 @@ def build(verbose: bool = False) -> str:
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB_PATH)
+    build_device_path()
     return LIB_PATH
+
+
+def build_device_path() -> str:
+    """The C caller of tests/c/device_path.c, linked against libqdeq.so (found
+    through an $ORIGIN-relative rpath) and the HIP runtime."""
+    cmd = ["gcc", "-std=c99", "-O2", "-Wall", "-D__HIP_PLATFORM_AMD__",
+           "-I", os.path.join(ROOT, "include"), "-I", os.path.join(ROCM, "include"),
+           DEVICE_PATH_SRC, "-o", DEVICE_PATH_BIN, "-L", HERE, "-lqdeq",
+           "-L", os.path.join(ROCM, "lib"), "-lamdhip64", "-lm",
+           "-Wl,-rpath,$ORIGIN/../qdeqcycle_fdm_amd:" + os.path.join(ROCM, "lib")]
+    subprocess.run(cmd, check=True)
+    return DEVICE_PATH_BIN
 
 
 if __name__ == "__main__":

@@ -24,7 +24,9 @@ Named workloads:
 from __future__ import annotations
 
 import dataclasses
+import json
 import math
+import os
 from typing import Optional
 
 import numpy as np
@@ -69,6 +71,8 @@ class SolverParams:
     V0: float = 1.0e-6
     L: float = 0.008
     Vini: float = 1.0e-9
+    max_steps: int = 100000000  # maxNumSteps, SetParams.cpp:18
+    it_save_jump: int = 2       # output record every it_save_jump steps, :27
     # engine-only knobs (no reference counterpart)
     precision: int = PREC_FP64
     cycle: int = CYCLE_REFERENCE
@@ -264,11 +268,153 @@ def ensemble_members(count: int = 64, seed: int = 0,
     return out
 
 
-WORKLOADS = {
-    "bp1_129": lambda: make_bp1_problem(129),
-    "basin_1025": lambda: make_bp1_problem(1025, basin=True, tol_mode=TOL_REL,
-                                           tol=1.0e-10),
-    "basin_2049": lambda: make_bp1_problem(2049, basin=True, tol_mode=TOL_REL,
-                                           tol=1.0e-10),
-    "ref_basin_801": make_reference_problem,
-}
+# ---------------------------------------------------------------------------
+# Parameter files
+# ---------------------------------------------------------------------------
+#
+# The reference sets its parameters in code (setparams, SetParams.cpp:3-30;
+# the friction and medium in fault.cpp:73-135 and odeSolver.cpp:272-319) and
+# its README announces a parameter-file reader.  A parameter file here is a
+# JSON object whose keys keep the reference's names where it has them:
+#
+#   "problem"      "bp1" | "basin" | "reference"  (required: which set-up)
+#   "ni", "nk", "dh"                      grid (GRID, SetParams.cpp:21-23);
+#                                         bp1/basin: ni = nk, dh = 48 km/(n-1)
+#   "t_year" .. "t_sec" or "tmax"         run length (SetParams.cpp:5-12)
+#   "ini_dt", "mindt", "maxdt", "totTol", "maxNumSteps", "it_save_jump"
+#   "f0", "V0", "Dc", "Vini"              friction constants (fault.cpp:85-89)
+#   "sigma_n", "b_seis"                   bp1/basin: normal stress, b above 15 km
+#   "n_levels", "n_cycles", "relax"       multigrid (odeSolver.cpp:2-4)
+#   "tol_mode" ("abs" | "rel"), "tol", "max_vcycles"
+#   "cycle" ("reference" | "fast"), "n_pre", "n_post", "n_coarse"
+#   "precision" ("fp64" | "fp32_ir")
+#   "u0_file"                             initial displacement in the
+#                                         reference's format (odeSolver.cpp:
+#                                         96-128), relative to the file
+#
+# Unknown keys and values of the wrong type are errors.
+
+_TIME_KEYS = ("t_year", "t_month", "t_day", "t_hour", "t_min", "t_sec")
+_INT_KEYS = {"ni", "nk", "maxNumSteps", "it_save_jump", "n_levels", "n_cycles",
+             "max_vcycles", "n_pre", "n_post", "n_coarse", *_TIME_KEYS}
+_FLOAT_KEYS = {"dh", "tmax", "ini_dt", "mindt", "maxdt", "totTol", "f0", "V0",
+               "Dc", "Vini", "sigma_n", "b_seis", "relax", "tol"}
+_STR_KEYS = {"problem": ("bp1", "basin", "reference"), "tol_mode": ("abs", "rel"),
+             "cycle": ("reference", "fast"), "precision": ("fp64", "fp32_ir"),
+             "u0_file": None, "name": None}
+
+
+class ParamsError(ValueError):
+    pass
+
+
+def _check_types(d: dict, where: str) -> None:
+    for k, v in d.items():
+        if k in _INT_KEYS:
+            if isinstance(v, bool) or not isinstance(v, int):
+                raise ParamsError(f"{where}: {k} must be an integer, got {v!r}")
+        elif k in _FLOAT_KEYS:
+            if isinstance(v, bool) or not isinstance(v, (int, float)):
+                raise ParamsError(f"{where}: {k} must be a number, got {v!r}")
+        elif k in _STR_KEYS:
+            allowed = _STR_KEYS[k]
+            if not isinstance(v, str) or (allowed and v not in allowed):
+                raise ParamsError(f"{where}: {k} must be one of {allowed}, got {v!r}")
+        else:
+            raise ParamsError(f"{where}: unknown parameter {k!r}")
+
+
+def problem_from_params(d: dict, base_dir: str = ".", where: str = "parameters") -> Problem:
+    """Builds the Problem a parameter dictionary describes (keys above)."""
+    _check_types(d, where)
+    if "problem" not in d:
+        raise ParamsError(f"{where}: 'problem' is required")
+    kind = d["problem"]
+    if any(k in d for k in _TIME_KEYS) and "tmax" in d:
+        raise ParamsError(f"{where}: give either tmax or t_year..t_sec")
+    if kind == "reference":
+        for k in ("sigma_n", "b_seis"):
+            if k in d:
+                raise ParamsError(f"{where}: {k} is fixed by the reference problem")
+        ni, nk, dh = d.get("ni", 801), d.get("nk", 801), float(d.get("dh", 30.0))
+        u0 = None
+        if "u0_file" in d:
+            u0 = read_reference_u0(os.path.join(base_dir, d["u0_file"]), ni, nk)
+        pb = make_reference_problem(ni, nk, dh, U0=u0)
+    else:
+        n = d.get("ni", d.get("nk", 129))
+        if d.get("nk", n) != n:
+            raise ParamsError(f"{where}: the {kind} problem is square (ni = nk)")
+        if "dh" in d and abs(float(d["dh"]) - 48000.0 / (n - 1)) > 1e-9:
+            raise ParamsError(f"{where}: dh is fixed at 48 km / (n - 1) = "
+                              f"{48000.0 / (n - 1)} m for the {kind} problem")
+        if "u0_file" in d:
+            raise ParamsError(f"{where}: u0_file applies to the reference problem")
+        pb = make_bp1_problem(n, levels=d.get("n_levels"), basin=kind == "basin",
+                              Dc=float(d.get("Dc", 0.008)),
+                              sigma_n=float(d.get("sigma_n", 50.0)),
+                              b_seis=None if "b_seis" not in d else float(d["b_seis"]))
+    p = pb.params
+    upd = {}
+    if any(k in d for k in _TIME_KEYS):
+        t = {k: d.get(k, 0) for k in _TIME_KEYS}
+        upd["tmax"] = t["t_sec"] + 60.0 * (t["t_min"] + 60.0 * (t["t_hour"] + 24.0 * (
+            t["t_day"] + 30.0 * (t["t_month"] + 12.0 * t["t_year"]))))
+    names = {"tmax": "tmax", "ini_dt": "ini_dt", "mindt": "mindt", "maxdt": "maxdt",
+             "totTol": "totTol", "maxNumSteps": "max_steps",
+             "it_save_jump": "it_save_jump", "f0": "f0", "V0": "V0", "Dc": "L",
+             "Vini": "Vini", "n_levels": "n_levels", "n_cycles": "n_cycles",
+             "relax": "relax", "tol": "tol", "max_vcycles": "max_vcycles",
+             "n_pre": "n_pre", "n_post": "n_post", "n_coarse": "n_coarse"}
+    for k, field in names.items():
+        if k in d:
+            upd[field] = type(getattr(p, field))(d[k])
+    if "tol_mode" in d:
+        upd["tol_mode"] = TOL_ABS if d["tol_mode"] == "abs" else TOL_REL
+    if "cycle" in d:
+        upd["cycle"] = CYCLE_REFERENCE if d["cycle"] == "reference" else CYCLE_FAST
+    if "precision" in d:
+        upd["precision"] = PREC_FP64 if d["precision"] == "fp64" else PREC_FP32_IR
+    params = dataclasses.replace(p, **upd)
+    if params.ini_dt <= 0 or params.mindt <= 0 or params.maxdt < params.mindt:
+        raise ParamsError(f"{where}: need 0 < mindt <= maxdt and ini_dt > 0")
+    if params.tol <= 0 or params.totTol <= 0 or params.max_vcycles < 1:
+        raise ParamsError(f"{where}: tol, totTol and max_vcycles must be positive")
+    n = min(params.ni, params.nk)
+    for _ in range(params.n_levels - 1):
+        if n < 3:
+            raise ParamsError(f"{where}: too many levels ({params.n_levels}) for the grid")
+        n = (n - 1) // 2 + 1
+    if n < 3:
+        raise ParamsError(f"{where}: too many levels ({params.n_levels}) for the grid")
+    name = d.get("name") or pb.name
+    return dataclasses.replace(pb, params=params, name=name)
+
+
+def load_params_file(path: str) -> Problem:
+    """Reads a JSON parameter file (keys above) into a Problem."""
+    with open(path) as fh:
+        try:
+            d = json.load(fh)
+        except json.JSONDecodeError as e:
+            raise ParamsError(f"{path}: not valid JSON ({e})") from None
+    if not isinstance(d, dict):
+        raise ParamsError(f"{path}: expected a JSON object")
+    return problem_from_params(d, os.path.dirname(os.path.abspath(path)), path)
+
+
+PARAMS_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "params")
+
+
+def workload(name: str) -> Problem:
+    """A named workload from the package's parameter files (params/*.json):
+    bp1_129, basin_1025, basin_2049, ref_basin_801."""
+    path = os.path.join(PARAMS_DIR, name + ".json")
+    if not os.path.exists(path):
+        have = sorted(f[:-5] for f in os.listdir(PARAMS_DIR) if f.endswith(".json"))
+        raise KeyError(f"unknown workload {name!r}; have {have}")
+    return load_params_file(path)
+
+
+WORKLOADS = {n: (lambda n=n: workload(n)) for n in
+             ("bp1_129", "basin_1025", "basin_2049", "ref_basin_801")}

@@ -30,6 +30,8 @@ int set_error(qdeq_ctx *c, int code, const std::string &msg) {
 
 using namespace qdeq;
 
+static int begin_call(qdeq_ctx *c, int where);
+
 template <typename T>
 static hipError_t alloc_zero(T **p, size_t n, hipStream_t s) {
   hipError_t e = hipMalloc(p, n * sizeof(T));
@@ -70,6 +72,15 @@ const char *qdeq_last_error(const qdeq_ctx *ctx) {
 
 const char *qdeq_global_error(void) { return g_error.c_str(); }
 
+int qdeq_hip_runtime_version(int *built, int *running) {
+  if (built) *built = HIP_VERSION;
+  if (!running) return QDEQ_OK;
+  *running = 0;
+  const hipError_t e = hipRuntimeGetVersion(running);
+  if (e != hipSuccess) return set_error(nullptr, QDEQ_ERR_HIP, hipGetErrorString(e));
+  return QDEQ_OK;
+}
+
 static void free_ctx(qdeq_ctx *c) {
   if (!c) return;
   mg_destroy_graphs(c);
@@ -98,6 +109,7 @@ static void free_ctx(qdeq_ctx *c) {
   (void)qdeq_output_close(c);
   events_free(c);
   if (c->rk.ev) (void)hipEventDestroy(c->rk.ev);
+  if (c->ev_caller) (void)hipEventDestroy(c->ev_caller);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -183,7 +195,7 @@ int qdeq_create(const qdeq_config *cfg, const double *mu,
       return QDEQ_ERR_HIP;
     }
   }
-  CREATE_HIP(alloc_zero(&c->partial, 16384, c->stream));
+  CREATE_HIP(alloc_zero(&c->partial, kPartialCap, c->stream));
   CREATE_HIP(alloc_zero(&c->scal, SC_COUNT, c->stream));
   CREATE_HIP(alloc_zero(&c->flags, 4, c->stream));
   CREATE_HIP(hipHostMalloc(&c->scal_host, SC_COUNT * sizeof(double)));
@@ -261,6 +273,15 @@ int qdeq_set_tiles(qdeq_ctx *c, int tiled, int tile_x, int tile_y) {
     return set_error(c, QDEQ_ERR_ARG, "tile sizes must be even, 8..256 x 2..128");
   if (!tiles_fit(c, tile_x, tile_y))
     return set_error(c, QDEQ_ERR_ARG, "tile too large for the LDS at these sweep counts");
+  if (c->cfg.precision == QDEQ_PREC_FP64) {
+    // the FP64 cycle's level-0 up kernel writes one norm partial per tile
+    const LevelDims &d = c->lv[0];
+    const long blocks = (long)((d.K + tile_x - 1) / tile_x) * ((d.N + tile_y - 1) / tile_y);
+    if (blocks > kPartialCap)
+      return set_error(c, QDEQ_ERR_ARG,
+                       "tile too small: " + std::to_string(blocks) +
+                           " level-0 tiles, at most " + std::to_string(kPartialCap));
+  }
   c->tiled = tiled != 0;
   c->tile_tx = tile_x;
   c->tile_ty = tile_y;
@@ -366,6 +387,7 @@ int qdeq_copy_probe(qdeq_ctx *c, long long n_doubles, int reps, double *avg_ms) 
 
 int qdeq_get_mu_equi(qdeq_ctx *c, double *me, int where) {
   if (!c || !me) return set_error(c, QDEQ_ERR_ARG, "null argument");
+  if (int st = begin_call(c, where)) return st;
   const size_t nme = (size_t)(2 * c->nx - 1) * (2 * c->nz - 1);
   QDEQ_HIP(c, hipMemcpyAsync(me, c->mu_equi, nme * sizeof(double),
                              where == QDEQ_HOST ? hipMemcpyDeviceToHost
@@ -409,6 +431,20 @@ int qdeq_build_mu_equi(int ni, int nk, const double *mu, double *me,
 
 // ---- drop-in entry points ----
 
+// Arrays passed as QDEQ_DEVICE may have been written by the caller's work on
+// the null stream; the context's own stream does not wait for that stream by
+// itself (it is non-blocking), so it waits for an event recorded there.
+static int begin_call(qdeq_ctx *c, int where) {
+  if (where != QDEQ_HOST && where != QDEQ_DEVICE)
+    return set_error(c, QDEQ_ERR_ARG, "pointer space must be QDEQ_HOST or QDEQ_DEVICE");
+  if (where == QDEQ_HOST) return QDEQ_OK;
+  if (!c->ev_caller)
+    QDEQ_HIP(c, hipEventCreateWithFlags(&c->ev_caller, hipEventDisableTiming));
+  QDEQ_HIP(c, hipEventRecord(c->ev_caller, nullptr));
+  QDEQ_HIP(c, hipStreamWaitEvent(c->stream, c->ev_caller, 0));
+  return QDEQ_OK;
+}
+
 static int to_device(qdeq_ctx *c, const double *src, double *stage, size_t n,
                      int where, const double **dst) {
   if (where == QDEQ_DEVICE) {
@@ -434,7 +470,9 @@ int qdeq_ode_solve(qdeq_ctx *c, double *U, int where, int *vcycles,
   if (!c || !U) return set_error(c, QDEQ_ERR_ARG, "null argument");
   const size_t ng = (size_t)c->nx * c->nz;
   const double *Ud;
-  int st = to_device(c, U, c->stage_U, ng, where, &Ud);
+  int st = begin_call(c, where);
+  if (st) return st;
+  st = to_device(c, U, c->stage_U, ng, where, &Ud);
   if (st) return st;
   double *Uw = const_cast<double *>(Ud);
   launch_pack(c, Uw);
@@ -454,6 +492,7 @@ int qdeq_set_boundary(qdeq_ctx *c, double *U, const double *slip,
   const size_t ng = (size_t)c->nx * c->nz, nz = (size_t)c->nz;
   const double *Ud, *sd, *ssd;
   int st;
+  if ((st = begin_call(c, where))) return st;
   if ((st = to_device(c, U, c->stage_U, ng, where, &Ud))) return st;
   if ((st = to_device(c, slip, c->stage_v[0], nz, where, &sd))) return st;
   if ((st = to_device(c, ss, c->stage_v[1], nz, where, &ssd))) return st;
@@ -471,6 +510,7 @@ int qdeq_eval_stress(qdeq_ctx *c, const double *U, const double *mu,
   const size_t ng = (size_t)c->nx * c->nz, nz = (size_t)c->nz;
   const double *Ud, *mud, *td;
   int st;
+  if ((st = begin_call(c, where))) return st;
   if ((st = to_device(c, U, c->stage_U, ng, where, &Ud))) return st;
   if ((st = to_device(c, mu, c->stage_U + ng, ng, where, &mud))) return st;
   if ((st = to_device(c, tau_n0, c->stage_v[0], nz, where, &td))) return st;
@@ -499,6 +539,7 @@ int qdeq_nr_search(qdeq_ctx *c, const double *Ts, const double *Tn,
   const double *p[5];
   const double *src[5] = {Ts, Tn, psi, a, eta};
   int st;
+  if ((st = begin_call(c, where))) return st;
   for (int i = 0; i < 5; i++)
     if ((st = to_device(c, src[i], c->stage_v[i], nz, where, &p[i]))) return st;
   const double *Vd0;
@@ -523,6 +564,7 @@ int qdeq_cal_dpsi(qdeq_ctx *c, const double *psi, const double *V,
   const size_t nz = (size_t)c->nz;
   const double *pd, *vd, *bd;
   int st;
+  if ((st = begin_call(c, where))) return st;
   if ((st = to_device(c, psi, c->stage_v[0], nz, where, &pd))) return st;
   if ((st = to_device(c, V, c->stage_v[1], nz, where, &vd))) return st;
   if ((st = to_device(c, b, c->stage_v[2], nz, where, &bd))) return st;

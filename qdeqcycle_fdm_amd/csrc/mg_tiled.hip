@@ -900,7 +900,6 @@ static void add_fitted(std::vector<int> &v, int n, const int *parts, int np) {
     if (t >= 2 && std::find(v.begin(), v.end(), t) == v.end()) v.push_back(t);
   }
 }
-constexpr int kPartialCap = 16384;  // c->partial entries (level-0 norm blocks)
 
 template <typename T, bool EXACT>
 static int autotune_t(qdeq_ctx *c, LevelBufs<T> *B, bool level0_has_f,

@@ -168,6 +168,7 @@ struct qdeq_ctx {
   int timing = 0;
   bool emit_events = false;
   hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;
+  hipEvent_t ev_caller = nullptr;  // the caller's null-stream work (QDEQ_DEVICE)
   double timed_ms = 0.0;
   long long timed_calls = 0, timed_seq = 0;
   // learned initial guess (guess.hip): attached network, mode, the response
@@ -309,6 +310,11 @@ __device__ inline void coarse_matvec(const float *__restrict__ minvT, const T *g
 }
 
 int build_coarse_inverse(qdeq_ctx *c);
+
+// Entries of c->partial: the FP64 fast cycle's level-0 up kernel writes one
+// norm partial per tile, so a level-0 tiling may have at most this many
+// tiles (qdeq_set_tiles, set_tile_shapes and the autotuner enforce it).
+constexpr int kPartialCap = 16384;
 
 // Row-partial layout of the level-0 residual norms (mg.hip): one block per
 // row, at most kNormCap blocks; plain norms at partial[0..), boundary data

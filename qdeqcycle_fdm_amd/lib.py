@@ -33,7 +33,8 @@ QDEQ_HOST, QDEQ_DEVICE = 0, 1
 # Every entry point include/qdeq.h declares.
 EXPORTS = (
     "qdeq_default_config", "qdeq_create", "qdeq_destroy", "qdeq_last_error",
-    "qdeq_global_error", "qdeq_get_mu_equi", "qdeq_build_mu_equi",
+    "qdeq_global_error", "qdeq_hip_runtime_version", "qdeq_get_mu_equi",
+    "qdeq_build_mu_equi",
     "qdeq_ode_solve", "qdeq_set_boundary", "qdeq_eval_stress",
     "qdeq_nr_search", "qdeq_cal_dpsi", "qdeq_compute_error",
     "qdeq_compute_step_size", "qdeq_rk43_setup", "qdeq_rk43_step",
@@ -129,8 +130,24 @@ _vp = ctypes.c_void_p
 _lib = None
 
 
+def check_runtime(built: int, running: int) -> None:
+    """Refuses a HIP runtime other than the one the library was built
+    against (major.minor, HIP_VERSION encoding).  A library loaded earlier
+    can bring its own libamdhip64 under the same soname (torch's wheels
+    bundle one), and libqdeq then binds to that runtime."""
+    def mm(v):
+        return v // 10000000, (v // 100000) % 100
+    if mm(built) != mm(running):
+        raise RuntimeError(
+            "libqdeq.so was built against HIP %d.%d but this process runs HIP "
+            "%d.%d, loaded by another library under the same soname (for "
+            "example torch's bundled runtime): load qdeqcycle_fdm_amd.lib before "
+            "that library, or run without it" % (*mm(built), *mm(running)))
+
+
 def load(path: str = LIB_PATH):
-    """Loads libqdeq.so (raises if it has not been built)."""
+    """Loads libqdeq.so (raises if it has not been built, or if the process
+    runs another HIP runtime than the one it was built against)."""
     global _lib
     if _lib is not None:
         return _lib
@@ -138,6 +155,10 @@ def load(path: str = LIB_PATH):
         raise FileNotFoundError(
             f"{path} is missing: build it with qdeqcycle_fdm_amd.build.build()")
     L = ctypes.CDLL(path)
+    L.qdeq_hip_runtime_version.argtypes = [_ip, _ip]
+    built, running = ctypes.c_int(0), ctypes.c_int(0)
+    if L.qdeq_hip_runtime_version(ctypes.byref(built), ctypes.byref(running)) == QDEQ_OK:
+        check_runtime(built.value, running.value)
     L.qdeq_default_config.argtypes = [ctypes.POINTER(QdeqConfig)]
     L.qdeq_default_config.restype = None
     L.qdeq_create.argtypes = [ctypes.POINTER(QdeqConfig), _dp, _dp,
@@ -234,15 +255,84 @@ def _p(a: np.ndarray):
     return a.ctypes.data_as(_dp)
 
 
-def _vptr(a):
-    """Pointer of a numpy array (host) or a torch tensor (device)."""
-    if isinstance(a, np.ndarray):
-        assert a.dtype == np.float64 and a.flags.c_contiguous
-        return ctypes.c_void_p(a.ctypes.data), QDEQ_HOST
-    # torch tensor (device memory when a.is_cuda)
-    if str(a.dtype) != "torch.float64" or not a.is_contiguous():
-        raise TypeError("expected a contiguous float64 array or tensor")
-    return ctypes.c_void_p(a.data_ptr()), QDEQ_DEVICE if a.is_cuda else QDEQ_HOST
+def _hip():
+    """The HIP runtime libqdeq.so runs on (already loaded, same soname)."""
+    load()
+    return ctypes.CDLL("libamdhip64.so.7")
+
+
+class DeviceArray:
+    """A contiguous FP64 array in device memory, for the QDEQ_DEVICE path of
+    the drop-in entry points.  Allocated with hipMalloc of the runtime
+    libqdeq.so runs on; copies are synchronous."""
+
+    def __init__(self, shape, device: int = 0):
+        self.shape = tuple(int(n) for n in np.atleast_1d(shape))
+        self.size = int(np.prod(self.shape))
+        self.device = device
+        self.ptr = ctypes.c_void_p()
+        hip = _hip()
+        if hip.hipSetDevice(device) != 0 or \
+                hip.hipMalloc(ctypes.byref(self.ptr), ctypes.c_size_t(8 * self.size)) != 0:
+            self.ptr = ctypes.c_void_p()
+            raise QdeqError(QDEQ_ERR_HIP, f"hipMalloc of {self.size} doubles failed")
+
+    @classmethod
+    def from_host(cls, a, device: int = 0) -> "DeviceArray":
+        a = _c(a)
+        d = cls(a.shape, device)
+        d.copy_from(a)
+        return d
+
+    def copy_from(self, a) -> None:
+        a = _c(a)
+        if a.size != self.size:
+            raise ValueError(f"copy of {a.size} values into {self.size}")
+        if _hip().hipMemcpy(self.ptr, ctypes.c_void_p(a.ctypes.data),
+                            ctypes.c_size_t(8 * self.size), 1) != 0:
+            raise QdeqError(QDEQ_ERR_HIP, "hipMemcpy to the device failed")
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty(self.shape)
+        if _hip().hipMemcpy(ctypes.c_void_p(out.ctypes.data), self.ptr,
+                            ctypes.c_size_t(8 * self.size), 2) != 0:
+            raise QdeqError(QDEQ_ERR_HIP, "hipMemcpy to the host failed")
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            _hip().hipFree(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _vptrs(arrays, sizes, device: int):
+    """Pointers and the common pointer space of the arrays of one call: all
+    numpy FP64 arrays (QDEQ_HOST) or all DeviceArrays on `device`
+    (QDEQ_DEVICE), each with the number of values the entry point reads."""
+    host = [isinstance(a, np.ndarray) for a in arrays]
+    dev = [isinstance(a, DeviceArray) for a in arrays]
+    if not (all(host) or all(dev)):
+        raise TypeError("the arrays of one call must all be numpy float64 arrays "
+                        "or all DeviceArrays")
+    out = []
+    for a, n in zip(arrays, sizes):
+        if a.size != n:
+            raise ValueError(f"array of {a.size} values where {n} are needed")
+        if all(host):
+            if a.dtype != np.float64 or not a.flags.c_contiguous:
+                raise TypeError("expected contiguous float64 arrays")
+            out.append(ctypes.c_void_p(a.ctypes.data))
+        else:
+            if a.device != device:
+                raise ValueError(f"array on device {a.device}, context on {device}")
+            out.append(a.ptr)
+    return out, (QDEQ_HOST if all(host) else QDEQ_DEVICE)
 
 
 def make_config(params: cfgmod.SolverParams, device: int = 0) -> QdeqConfig:
@@ -409,9 +499,9 @@ class Engine:
 
     # -- drop-in functions (reference names) --
     def ode_solve(self, U):
-        """ode_solve (odeSolver.cpp:605): in place on U (nz, nx); returns
-        (vcycles, res0, res)."""
-        ptr, where = _vptr(U)
+        """ode_solve (odeSolver.cpp:605): in place on U (nz, nx), a numpy
+        array or a DeviceArray; returns (vcycles, res0, res)."""
+        (ptr,), where = _vptrs([U], [self.nx * self.nz], self.cfg.device)
         vc = ctypes.c_int(0)
         r0 = ctypes.c_double(0)
         r = ctypes.c_double(0)
@@ -420,9 +510,11 @@ class Engine:
         return vc.value, r0.value, r.value
 
     def setBoundary(self, U, slip, slip_shift, t, ratio, dt):
-        pu, where = _vptr(U)
-        ps, _ = _vptr(slip)
-        pss, _ = _vptr(slip_shift)
+        """setBoundary (odeSolver.cpp:1150): in place on U; U, slip and
+        slip_shift all numpy arrays or all DeviceArrays."""
+        (pu, ps, pss), where = _vptrs([U, slip, slip_shift],
+                                      [self.nx * self.nz, self.nz, self.nz],
+                                      self.cfg.device)
         self._check(load().qdeq_set_boundary(self.h, pu, ps, pss, t, ratio, dt, where))
 
     def eval_stress_on_fault(self, U, mu, tau_n0):

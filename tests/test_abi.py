@@ -60,3 +60,53 @@ def test_header_compiles_as_c():
     p = subprocess.run(["gcc", "-std=c99", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
                         "-x", "c", "-"], input=src, capture_output=True, text=True)
     assert p.returncode == 0, p.stderr
+
+
+def test_hip_runtime_version_check():
+    """The library reports the HIP version it was built against and the one
+    the process runs; lib.load() refuses a different major.minor."""
+    if not os.path.exists(lib.LIB_PATH):
+        pytest.skip("libqdeq.so not built")
+    import ctypes
+    L = lib.load()
+    built, running = ctypes.c_int(0), ctypes.c_int(0)
+    assert L.qdeq_hip_runtime_version(ctypes.byref(built), ctypes.byref(running)) == 0
+    hdr = open("/opt/rocm/include/hip/hip_version.h").read()
+    v = {k: int(re.search(rf"#define HIP_VERSION_{k} (\d+)", hdr).group(1))
+         for k in ("MAJOR", "MINOR", "PATCH")}
+    assert built.value == v["MAJOR"] * 10000000 + v["MINOR"] * 100000 + v["PATCH"]
+    assert running.value // 100000 == built.value // 100000
+    lib.check_runtime(70226015, 70226999)           # same 7.2: accepted
+    with pytest.raises(RuntimeError, match=r"built against HIP 7\.2 .* runs HIP 7\.0"):
+        lib.check_runtime(70226015, 70051831)
+
+
+def test_load_after_another_hip_runtime_is_refused():
+    """torch's wheel bundles its own libamdhip64 under the same soname; a
+    process that imported torch first runs that runtime, and loading the
+    library there raises instead of running on it."""
+    if not os.path.exists(lib.LIB_PATH):
+        pytest.skip("libqdeq.so not built")
+    code = ("import torch\n"
+            "print('TORCH_HIP', torch.version.hip)\n"
+            "from qdeqcycle_fdm_amd import lib\n"
+            "try:\n"
+            "    lib.load()\n"
+            "    print('LOADED')\n"
+            "except RuntimeError as e:\n"
+            "    print('REFUSED', e)\n")
+    out = subprocess.run([os.environ.get("PYTHON", "python3"), "-c", code], cwd=ROOT,
+                         capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stderr
+    tv = re.search(r"TORCH_HIP (\S+)", out.stdout).group(1)
+    if tv.startswith("7.2"):
+        assert "LOADED" in out.stdout
+    else:
+        assert "REFUSED" in out.stdout and "load qdeqcycle_fdm_amd.lib before" in out.stdout
+
+
+def test_device_path_program_builds():
+    """The C caller of the device path (run by tests/test_gpu_device_path.py)
+    is built with the library."""
+    from qdeqcycle_fdm_amd import build
+    assert os.path.exists(build.DEVICE_PATH_BIN) or not os.path.exists(lib.LIB_PATH)
