@@ -168,6 +168,19 @@ int qdeq_get_mu_equi(qdeq_ctx *ctx, double *mu_equi, int where);
 int qdeq_build_mu_equi(int ni, int nk, const double *mu, double *mu_equi,
                        int device);
 
+/* The same build that allocates nothing (no reference counterpart; for
+ * callers that cannot allocate device memory, such as loops that rebuild
+ * the medium of many ensemble members without touching the allocator): mu
+ * (nodal nx*nz, ghost entries ignored) and mu_equi ((2nx-1)*(2nz-1)) are
+ * device arrays on `device`; the work runs on the null stream and the call
+ * returns when it is done.  The interpolation weight tables live in a
+ * fixed 76,800-byte device array that each device keeps until the process
+ * ends; every call recomputes them, and calls that overlap on one device
+ * race on it.  Grids of 3..65535 nodes per side; the result equals
+ * qdeq_build_mu_equi's bit for bit. */
+int qdeq_build_mu_equi_noalloc(int ni, int nk, const double *mu,
+                               double *mu_equi, int device);
+
 /* ---- drop-in replacements of the reference's hot-path functions ---- */
 
 /* ode_solve (odeSolver.cpp:605-609) = multigrid (652-720) + extend_crew

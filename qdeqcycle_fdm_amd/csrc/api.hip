@@ -429,6 +429,26 @@ int qdeq_build_mu_equi(int ni, int nk, const double *mu, double *me,
   return QDEQ_OK;
 }
 
+// Largest grid side of the allocation-free mu_equi build: its face kernels
+// launch one grid row per node row, and a launch has at most 65535 rows.
+constexpr int kNoallocMaxSide = 65535;
+
+int qdeq_build_mu_equi_noalloc(int ni, int nk, const double *mu, double *me,
+                               int device) {
+  if (!mu || !me)
+    return set_error(nullptr, QDEQ_ERR_ARG, "qdeq_build_mu_equi_noalloc: null argument");
+  if (ni < 3 || nk < 3 || ni > kNoallocMaxSide || nk > kNoallocMaxSide)
+    return set_error(nullptr, QDEQ_ERR_ARG,
+                     "qdeq_build_mu_equi_noalloc: grid " + std::to_string(ni) + " x " +
+                         std::to_string(nk) + " outside 3.." +
+                         std::to_string(kNoallocMaxSide) + " nodes per side");
+  std::string msg;
+  const int st = build_mu_equi_noalloc(ni, nk, mu, me, device, &msg);
+  if (st != QDEQ_OK)
+    return set_error(nullptr, st, "qdeq_build_mu_equi_noalloc: " + msg);
+  return QDEQ_OK;
+}
+
 // ---- drop-in entry points ----
 
 // Arrays passed as QDEQ_DEVICE may have been written by the caller's work on

@@ -9,6 +9,7 @@
 // face.  The x-face block entry the reference leaves unassigned
 // (odeSolver.cpp:434-436 skip mu_matrix[1][1]) takes the nodal value at that
 // position.
+#include <mutex>
 #include <vector>
 
 #include "qdeq_internal.h"
@@ -17,41 +18,62 @@ namespace qdeq {
 
 constexpr int NSAMP = 400;
 
-// Weights w[s][i*nj + j] = Lx[i] * Ly[j] for sample s (mj outer, mi inner,
-// both in -10..10 without 0), as interp_2d forms them (odeSolver.cpp:134-200).
-static void lagrange_table(const double *xn, const double *yn, int ni, int nj,
-                           std::vector<double> &w) {
-  w.assign((size_t)NSAMP * ni * nj, 0.0);
+// Sample positions of the interpolation nodes: 3 nodes at -1, 0, 1 or 4 at
+// -1.5, -0.5, 0.5, 1.5 (odeSolver.cpp:423-424, 466-467), exact in binary.
+__host__ __device__ inline double node_pos(int n, int i) {
+  return n == 3 ? (double)(i - 1) : (double)i - 1.5;
+}
+
+// Weights w[i*NJ + j] = Lx[i] * Ly[j] of sample s (0..399): mj outer, mi
+// inner, both in -10..10 without 0, as interp_2d forms them
+// (odeSolver.cpp:134-200).  Host and device evaluate the same operations in
+// the same order, so the two tables hold the same bits.
+template <int NI, int NJ>
+__host__ __device__ inline void lagrange_sample(int s, double *w) {
   const int NG = 10;
-  int s = 0;
-  for (int mj = -NG; mj <= NG; mj++) {
-    if (mj == 0) continue;
-    for (int mi = -NG; mi <= NG; mi++) {
-      if (mi == 0) continue;
-      const double xi = mi / ((NG + 1.0) * 2.0);
-      const double yi = mj / ((NG + 1.0) * 2.0);
-      double Lx[4], Ly[4], t[4], b[4];
-      for (int i = 0; i < ni; i++) {
-        Lx[i] = 1.0;
-        for (int k = 0; k < ni; k++) b[k] = xn[i] - xn[k];
-        b[i] = 1.0;
-        for (int k = 0; k < ni; k++) t[k] = xi - xn[k];
-        t[i] = 1.0;
-        for (int k = 0; k < ni; k++) Lx[i] = Lx[i] * t[k] / b[k];
-      }
-      for (int i = 0; i < nj; i++) {
-        Ly[i] = 1.0;
-        for (int k = 0; k < nj; k++) b[k] = yn[i] - yn[k];
-        b[i] = 1.0;
-        for (int k = 0; k < nj; k++) t[k] = yi - yn[k];
-        t[i] = 1.0;
-        for (int k = 0; k < nj; k++) Ly[i] = Ly[i] * t[k] / b[k];
-      }
-      for (int i = 0; i < ni; i++)
-        for (int j = 0; j < nj; j++) w[(size_t)s * ni * nj + i * nj + j] = Lx[i] * Ly[j];
-      s++;
-    }
+  const int a = s / (2 * NG), c = s - a * (2 * NG);
+  const int mj = a < NG ? a - NG : a - NG + 1;
+  const int mi = c < NG ? c - NG : c - NG + 1;
+  const double xi = mi / ((NG + 1.0) * 2.0);
+  const double yi = mj / ((NG + 1.0) * 2.0);
+  double Lx[NI], Ly[NJ], t[4], b[4];
+  for (int i = 0; i < NI; i++) {
+    Lx[i] = 1.0;
+    for (int k = 0; k < NI; k++) b[k] = node_pos(NI, i) - node_pos(NI, k);
+    b[i] = 1.0;
+    for (int k = 0; k < NI; k++) t[k] = xi - node_pos(NI, k);
+    t[i] = 1.0;
+    for (int k = 0; k < NI; k++) Lx[i] = Lx[i] * t[k] / b[k];
   }
+  for (int i = 0; i < NJ; i++) {
+    Ly[i] = 1.0;
+    for (int k = 0; k < NJ; k++) b[k] = node_pos(NJ, i) - node_pos(NJ, k);
+    b[i] = 1.0;
+    for (int k = 0; k < NJ; k++) t[k] = yi - node_pos(NJ, k);
+    t[i] = 1.0;
+    for (int k = 0; k < NJ; k++) Ly[i] = Ly[i] * t[k] / b[k];
+  }
+  for (int i = 0; i < NI; i++)
+    for (int j = 0; j < NJ; j++) w[i * NJ + j] = Lx[i] * Ly[j];
+}
+
+// Both tables: x-faces (3 x 4 nodes) then z-faces (4 x 3 nodes), 400 x 12
+// doubles each.
+constexpr int kTableWords = NSAMP * 12;
+
+static void lagrange_tables_host(std::vector<double> &w) {
+  w.assign((size_t)2 * kTableWords, 0.0);
+  for (int s = 0; s < NSAMP; s++) {
+    lagrange_sample<3, 4>(s, &w[(size_t)s * 12]);
+    lagrange_sample<4, 3>(s, &w[(size_t)kTableWords + (size_t)s * 12]);
+  }
+}
+
+__global__ void k_lagrange_tables(double *__restrict__ w) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= NSAMP) return;
+  lagrange_sample<3, 4>(s, w + (size_t)s * 12);
+  lagrange_sample<4, 3>(s, w + (size_t)kTableWords + (size_t)s * 12);
 }
 
 // Harmonic mean over the samples of sum_j sum_i w z (odeSolver.cpp:442-460).
@@ -71,7 +93,9 @@ __device__ inline double face_mean(const double *__restrict__ w,
   return NSAMP / mu0;
 }
 
-// mh: nodal mu with the ghost layer filled, (nz, nx).  me: doubled array.
+// mh: nodal mu (nz, nx); only rows 1..nk, columns 1..ni are read, each ghost
+// taking the value of the nearest interior node (the ghost fill of
+// odeSolver.cpp:347-376).  me: doubled array.
 // kind 0: x-faces between nodes (i-1, i), rows 1..nk, i = 2..ni
 //      (odeSolver.cpp:421-462); kind 1: z-faces between rows (k-1, k),
 //      k = 2..nk, i = 2..ni-1 (464-506); kind 2/3: bottom/top boundary
@@ -84,7 +108,8 @@ __global__ void k_faces(const double *__restrict__ mh, double *__restrict__ me,
   const long W2 = 2L * nx - 1;
   const int i = blockIdx.x * blockDim.x + threadIdx.x + 2;
   const int r = blockIdx.y;
-#define MH(row, col) mh[(size_t)(row) * nx + (col)]
+#define MH(row, col) \
+  mh[(size_t)min(max((row), 1), nk) * nx + min(max((col), 1), ni)]
   if (kind == 0) {
     const int k = r + 1;
     if (i > ni || k > nk) return;
@@ -121,41 +146,45 @@ __global__ void k_faces(const double *__restrict__ mh, double *__restrict__ me,
 #undef MH
 }
 
-// Nodal entries of the doubled array, ghosts included (odeSolver.cpp:337-376).
+// Nodal entries of the doubled array, ghosts included (odeSolver.cpp:337-376):
+// a ghost takes the value of the nearest interior node.
 __global__ void k_nodal(const double *__restrict__ mh, double *__restrict__ me,
                         int nx, int nz) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int k = blockIdx.y;
   if (i >= nx || k >= nz) return;
-  me[2L * i + 2L * k * (2L * nx - 1)] = mh[(size_t)k * nx + i];
+  const int ic = min(max(i, 1), nx - 2), kc = min(max(k, 1), nz - 2);
+  me[2L * i + 2L * k * (2L * nx - 1)] = mh[(size_t)kc * nx + ic];
 }
 
-// mu_dev: nodal (nz, nx) mu on the device (ghost entries ignored).
+// The launches of the build: zeroed output, nodal entries, the four face
+// kinds.  mh: nodal mu on the device; w: both weight tables on the device.
+static void launch_mu_equi(int ni, int nk, const double *mh, double *me,
+                           const double *w, hipStream_t s) {
+  const int nx = ni + 2, nz = nk + 2;
+  const dim3 blk(64);
+  const unsigned gx = (unsigned)((ni + 63) / 64);
+  hipLaunchKernelGGL(k_nodal, dim3((unsigned)((nx + 255) / 256), nz), dim3(256),
+                     0, s, mh, me, nx, nz);
+  hipLaunchKernelGGL(k_faces, dim3(gx, nk), blk, 0, s, mh, me, w, w + kTableWords,
+                     ni, nk, 0);
+  hipLaunchKernelGGL(k_faces, dim3(gx, nk - 1), blk, 0, s, mh, me, w,
+                     w + kTableWords, ni, nk, 1);
+  hipLaunchKernelGGL(k_faces, dim3(gx, 1), blk, 0, s, mh, me, w, w + kTableWords,
+                     ni, nk, 2);
+  hipLaunchKernelGGL(k_faces, dim3(gx, 1), blk, 0, s, mh, me, w, w + kTableWords,
+                     ni, nk, 3);
+}
+
+// mu_host: nodal (nz, nx) mu on the host (ghost entries ignored).
 int build_mu_equi_device(int ni, int nk, const double *mu_host, double *me_dev,
                          hipStream_t s, std::string *err) {
   const int nx = ni + 2, nz = nk + 2;
-  // ghost fill of the nodal values (odeSolver.cpp:347-376)
-  std::vector<double> mh((size_t)nx * nz, 0.0);
-  for (int k = 1; k <= nk; k++)
-    for (int i = 1; i <= ni; i++) mh[(size_t)k * nx + i] = mu_host[(size_t)k * nx + i];
-  for (int i = 1; i <= ni; i++) {
-    mh[i] = mh[(size_t)1 * nx + i];
-    mh[(size_t)(nz - 1) * nx + i] = mh[(size_t)nk * nx + i];
-  }
-  for (int k = 0; k < nz; k++) {
-    mh[(size_t)k * nx] = mh[(size_t)k * nx + 1];
-    mh[(size_t)k * nx + nx - 1] = mh[(size_t)k * nx + ni];
-  }
-  static const double x_index[3] = {-1.0, 0.0, 1.0};
-  static const double y_index[4] = {-1.5, -0.5, 0.5, 1.5};
-  static const double x_index2[4] = {-1.5, -0.5, 0.5, 1.5};
-  static const double y_index2[3] = {-1.0, 0.0, 1.0};
-  std::vector<double> wx, wz;
-  lagrange_table(x_index, y_index, 3, 4, wx);
-  lagrange_table(x_index2, y_index2, 4, 3, wz);
-
+  std::vector<double> w;
+  lagrange_tables_host(w);
   double *d_mh = nullptr, *d_w = nullptr;
   const size_t nme = (size_t)(2 * nx - 1) * (2 * nz - 1);
+  const size_t nm = (size_t)nx * nz;
   hipError_t e;
 #define MQ_CHECK(call)                                               \
   do {                                                               \
@@ -167,32 +196,76 @@ int build_mu_equi_device(int ni, int nk, const double *mu_host, double *me_dev,
       return QDEQ_ERR_HIP;                                           \
     }                                                                \
   } while (0)
-  MQ_CHECK(hipMalloc(&d_mh, mh.size() * sizeof(double)));
-  MQ_CHECK(hipMalloc(&d_w, (wx.size() + wz.size()) * sizeof(double)));
-  MQ_CHECK(hipMemcpyAsync(d_mh, mh.data(), mh.size() * sizeof(double),
-                          hipMemcpyHostToDevice, s));
-  MQ_CHECK(hipMemcpyAsync(d_w, wx.data(), wx.size() * sizeof(double),
-                          hipMemcpyHostToDevice, s));
-  MQ_CHECK(hipMemcpyAsync(d_w + wx.size(), wz.data(), wz.size() * sizeof(double),
+  MQ_CHECK(hipMalloc(&d_mh, nm * sizeof(double)));
+  MQ_CHECK(hipMalloc(&d_w, w.size() * sizeof(double)));
+  MQ_CHECK(hipMemcpyAsync(d_mh, mu_host, nm * sizeof(double), hipMemcpyHostToDevice, s));
+  MQ_CHECK(hipMemcpyAsync(d_w, w.data(), w.size() * sizeof(double),
                           hipMemcpyHostToDevice, s));
   MQ_CHECK(hipMemsetAsync(me_dev, 0, nme * sizeof(double), s));
-  hipLaunchKernelGGL(k_nodal, dim3((unsigned)((nx + 255) / 256), nz), dim3(256),
-                     0, s, d_mh, me_dev, nx, nz);
-  const dim3 blk(64);
-  const unsigned gx = (unsigned)((ni + 63) / 64);
-  hipLaunchKernelGGL(k_faces, dim3(gx, nk), blk, 0, s, d_mh, me_dev, d_w,
-                     d_w + wx.size(), ni, nk, 0);
-  hipLaunchKernelGGL(k_faces, dim3(gx, nk - 1), blk, 0, s, d_mh, me_dev, d_w,
-                     d_w + wx.size(), ni, nk, 1);
-  hipLaunchKernelGGL(k_faces, dim3(gx, 1), blk, 0, s, d_mh, me_dev, d_w,
-                     d_w + wx.size(), ni, nk, 2);
-  hipLaunchKernelGGL(k_faces, dim3(gx, 1), blk, 0, s, d_mh, me_dev, d_w,
-                     d_w + wx.size(), ni, nk, 3);
+  launch_mu_equi(ni, nk, d_mh, me_dev, d_w, s);
   MQ_CHECK(hipGetLastError());
   MQ_CHECK(hipStreamSynchronize(s));
   (void)hipFree(d_mh);
   (void)hipFree(d_w);
 #undef MQ_CHECK
+  return QDEQ_OK;
+}
+
+// ---- the build without allocation ----
+//
+// The two weight tables live in a file-scope device array: each device the
+// library runs on holds its own copy until the process ends.  Host code
+// never uses the array's name as a device address; it looks the address up
+// once per device (hipGetSymbolAddress, with that device current) and passes
+// that.  Every call recomputes the tables into it, so nothing depends on an
+// earlier call; calls that overlap on one device share it and race.
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[2 * kTableWords];
+static double *g_scratch[kMaxDevices];
+static std::mutex g_scratch_mutex;
+
+static int scratch_on(int device, double **out, std::string *err) {
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) {
+      *err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return QDEQ_ERR_HIP;
+    }
+    g_scratch[device] = static_cast<double *>(p);
+  }
+  *out = g_scratch[device];
+  return QDEQ_OK;
+}
+
+int build_mu_equi_noalloc(int ni, int nk, const double *mu_dev, double *me_dev,
+                          int device, std::string *err) {
+  if (device < 0 || device >= kMaxDevices) {
+    *err = "device ordinal out of range";
+    return QDEQ_ERR_ARG;
+  }
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) {
+    *err = hipGetErrorString(e);
+    return QDEQ_ERR_HIP;
+  }
+  double *w = nullptr;
+  int st = scratch_on(device, &w, err);
+  if (st != QDEQ_OK) return st;
+  const size_t nme = (size_t)(2 * (ni + 2) - 1) * (2 * (nk + 2) - 1);
+  const hipStream_t s = nullptr;  // the null stream: no stream is created
+  hipLaunchKernelGGL(k_lagrange_tables, dim3((NSAMP + 127) / 128), dim3(128), 0, s, w);
+  e = hipMemsetAsync(me_dev, 0, nme * sizeof(double), s);
+  if (e == hipSuccess) {
+    launch_mu_equi(ni, nk, mu_dev, me_dev, w, s);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    *err = hipGetErrorString(e);
+    return QDEQ_ERR_HIP;
+  }
   return QDEQ_OK;
 }
 

@@ -330,9 +330,13 @@ inline bool launch_direct(const qdeq_ctx *c) {
 void launch_coarse_direct_fp64(qdeq_ctx *c, double *u, const double *f);
 void launch_coarse_direct_fp32(qdeq_ctx *c, float *u, const float *f);
 
-// medium (medium.hip)
-int build_mu_equi_device(int ni, int nk, const double *mu_dev, double *me_dev,
+// medium (medium.hip): from a host nodal array (allocates device staging),
+// and from a device nodal array into a device output without any
+// allocation (the weight tables in a file-scope device array)
+int build_mu_equi_device(int ni, int nk, const double *mu_host, double *me_dev,
                          hipStream_t s, std::string *err);
+int build_mu_equi_noalloc(int ni, int nk, const double *mu_dev, double *me_dev,
+                          int device, std::string *err);
 
 // fault + RK43 stages (fault.hip)
 void launch_set_boundary_grid(qdeq_ctx *c, double *U, const double *slip,

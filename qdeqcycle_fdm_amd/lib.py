@@ -34,7 +34,7 @@ QDEQ_HOST, QDEQ_DEVICE = 0, 1
 EXPORTS = (
     "qdeq_default_config", "qdeq_create", "qdeq_destroy", "qdeq_last_error",
     "qdeq_global_error", "qdeq_hip_runtime_version", "qdeq_get_mu_equi",
-    "qdeq_build_mu_equi",
+    "qdeq_build_mu_equi", "qdeq_build_mu_equi_noalloc",
     "qdeq_ode_solve", "qdeq_set_boundary", "qdeq_eval_stress",
     "qdeq_nr_search", "qdeq_cal_dpsi", "qdeq_compute_error",
     "qdeq_compute_step_size", "qdeq_rk43_setup", "qdeq_rk43_step",
@@ -171,6 +171,8 @@ def load(path: str = LIB_PATH):
     L.qdeq_get_mu_equi.argtypes = [_vp, _vp, ctypes.c_int]
     L.qdeq_build_mu_equi.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp,
                                      ctypes.c_int]
+    L.qdeq_build_mu_equi_noalloc.argtypes = [ctypes.c_int, ctypes.c_int, _vp, _vp,
+                                             ctypes.c_int]
     L.qdeq_ode_solve.argtypes = [_vp, _vp, ctypes.c_int, _ip, _dp, _dp]
     L.qdeq_set_boundary.argtypes = [_vp, _vp, _vp, _vp, ctypes.c_double,
                                     ctypes.c_double, ctypes.c_double, ctypes.c_int]
