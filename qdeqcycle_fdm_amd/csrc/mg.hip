@@ -127,15 +127,9 @@ __global__ __launch_bounds__(BX *BY) void k_prolong(
   }
 }
 
+// Workgroup sum (wavefront butterflies, wavefronts in order), every thread.
 __device__ inline double block_sum_256(double v, double *sh) {
-  const int tid = threadIdx.x;
-  sh[tid] = v;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (tid < s) sh[tid] += sh[tid + s];
-    __syncthreads();
-  }
-  return sh[0];
+  return block_sum1(v, sh);
 }
 
 // Residual of the level-0 operator at interior point (j, k), f = 0.
@@ -353,24 +347,13 @@ __global__ __launch_bounds__(NORM_THREADS) void k_start_partials(
       }
     }
   }
-  // the three block sums in one tree (each the order of block_sum_256)
-  const int tid = threadIdx.x;
-  sh[0][tid] = acc;
-  sh[1][tid] = accc;
-  sh[2][tid] = accb;
-  __syncthreads();
-  for (int st = NORM_THREADS / 2; st > 0; st >>= 1) {
-    if (tid < st) {
-      sh[0][tid] += sh[0][tid + st];
-      sh[1][tid] += sh[1][tid + st];
-      sh[2][tid] += sh[2][tid + st];
-    }
-    __syncthreads();
-  }
-  if (tid == 0) {
-    partial[blockIdx.x] = sh[0][0];
-    if (CAND) partial[2 * NORM_BLOCKS + blockIdx.x] = sh[1][0];
-    if (bd && blockIdx.x < BDRES_BLOCKS) partial[NORM_BLOCKS + blockIdx.x] = sh[2][0];
+  // the three block sums (each the order of block_sum_256)
+  double v3[3] = {acc, accc, accb};
+  block_sums<3>(v3, &sh[0][0]);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = v3[0];
+    if (CAND) partial[2 * NORM_BLOCKS + blockIdx.x] = v3[1];
+    if (bd && blockIdx.x < BDRES_BLOCKS) partial[NORM_BLOCKS + blockIdx.x] = v3[2];
   }
 }
 
@@ -393,19 +376,9 @@ __global__ __launch_bounds__(NORM_THREADS) void k_start_gate(
     for (int i = tid; i < BDRES_BLOCKS; i += NORM_THREADS) b += partial[NORM_BLOCKS + i];
   if (cand)
     for (int i = tid; i < nb; i += NORM_THREADS) cc += partial[2 * NORM_BLOCKS + i];
-  sh[0][tid] = a;
-  sh[1][tid] = b;
-  sh[2][tid] = cc;
-  __syncthreads();
-  for (int st = NORM_THREADS / 2; st > 0; st >>= 1) {
-    if (tid < st) {
-      sh[0][tid] += sh[0][tid + st];
-      sh[1][tid] += sh[1][tid + st];
-      sh[2][tid] += sh[2][tid + st];
-    }
-    __syncthreads();
-  }
-  const double su = sh[0][0], sb = sh[1][0], sc = sh[2][0];
+  double v3[3] = {a, b, cc};
+  block_sums<3>(v3, &sh[0][0]);
+  const double su = v3[0], sb = v3[1], sc = v3[2];
   if (threadIdx.x == 0) {
     const double du = sqrt(su), dc = sqrt(sc);
     const double r0 = rel ? sqrt(sb) : du;
@@ -814,7 +787,12 @@ int mg_begin(qdeq_ctx *c) {
     c->cand_in_u = false;
     const double *plain = in_u ? c->plain_interior : c->d[0].u;
     const double *cand = !m.cand ? nullptr : (in_u ? c->d[0].u : c->d[0].r);
-    if (cand)
+    // a plain start launched kernel by kernel: its norms and test ride on
+    // the first V-cycle's down kernels (the same decisions, no start launches)
+    const bool fused = !cand && launch_direct(c) && fused_start_fits(c);
+    if (fused)
+      c->fuse_start_next = true;
+    else if (cand)
       hipLaunchKernelGGL(k_start_partials<true>, dim3(norm_blocks(c)), dim3(NORM_THREADS), 0,
                          c->stream, plain, c->d[0].u, cand, c->d[0].cx, c->d[0].cz,
                          d.N, d.K, d.P, d.DH * d.DH, rel, c->partial);
@@ -822,9 +800,10 @@ int mg_begin(qdeq_ctx *c) {
       hipLaunchKernelGGL(k_start_partials<false>, dim3(norm_blocks(c)), dim3(NORM_THREADS), 0,
                          c->stream, plain, c->d[0].u, cand, c->d[0].cx, c->d[0].cz,
                          d.N, d.K, d.P, d.DH * d.DH, rel, c->partial);
-    hipLaunchKernelGGL(k_start_gate, dim3(1), dim3(NORM_THREADS), 0, c->stream,
-                       c->partial, norm_blocks(c), m.cand ? 1 : 0, rel, c->cfg.tol,
-                       c->cfg.max_vcycles, c->scal, c->mbox_dev, c->gate);
+    if (!fused)
+      hipLaunchKernelGGL(k_start_gate, dim3(1), dim3(NORM_THREADS), 0, c->stream,
+                         c->partial, norm_blocks(c), m.cand ? 1 : 0, rel, c->cfg.tol,
+                         c->cfg.max_vcycles, c->scal, c->mbox_dev, c->gate);
     if (in_u) {
       // the plain start back over a candidate that was not chosen
       hipLaunchKernelGGL(k_restore_if, dim3(std::min(d.N, 1024)), dim3(256), 0,

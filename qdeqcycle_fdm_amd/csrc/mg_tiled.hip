@@ -289,19 +289,110 @@ __host__ __device__ inline bool res_in_regs(int TX, int TY, int threads) {
   return (TX + 2) * (TY + 2) <= RES_REGS * threads;
 }
 
+// Start of a plain FP64 solve folded into its first V-cycle (no separate
+// start kernels): the level-0 down kernel (START_PARTIALS) forms, per tile
+// and before smoothing, the residual-norm partials of the start and of the
+// boundary data (the residual with the interior at zero), the expressions of
+// k_start_partials; the level-1 down kernel (START_TEST) sums them in a
+// fixed order (every workgroup the same bits), applies the start test of
+// k_start_gate, returns at once if the start already meets the threshold,
+// and its workgroup (0, 0) publishes the start as k_start_gate does.  Both
+// ignore the gate flag, which the previous solve left set.
+enum { DOWN_PLAIN = 0, START_PARTIALS = 1, START_TEST = 2 };
+// partials in c->partial: start norm at kStartOff + tile, boundary data at
+// kStartOff + kStartCap + tile (the level-0 up kernel's norm partials stay
+// below kStartOff)
+constexpr int kStartOff = 8192, kStartCap = 4096;
+static_assert(kStartOff + 2 * kStartCap <= kPartialCap, "start partials");
+
+struct StartArgs {
+  double *part;    // c->partial + kStartOff
+  int nb;          // tiles of the level-0 down kernel (partials per kind)
+  double *scal, *mb;
+  int *gate;
+  int rel, cap;
+  double tol;
+};
+
 // Pre-smoothing + residual + restriction (levels above the tail).
-template <typename T, bool HAS_F, bool EXACT>
+template <typename T, bool HAS_F, bool EXACT, int SMODE>
 __global__ __launch_bounds__(TILE_THREADS) void k_down(
     const T *__restrict__ uin, T *__restrict__ uout, const T *__restrict__ f,
     const T *__restrict__ cx, const T *__restrict__ cz, int N, int K, int P,
     T DH2, T relax, int nu, int zero_u, T *__restrict__ r2, int Nc, int Kc,
-    int Pc, int TX, int TY, const int *__restrict__ gate) {
-  if (gate && gate[0]) return;  // the solve has converged: skip this cycle
+    int Pc, int TX, int TY, const int *__restrict__ gate, StartArgs sa) {
+  if (SMODE == DOWN_PLAIN && gate && gate[0]) return;  // converged: skip
   extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ double sdec[16 * 2];
+  if (SMODE == START_TEST && threadIdx.y == 0) {
+    // wavefront 0: the start norms from the level-0 tiles' partials
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < sa.nb; i += 64) {
+      a += sa.part[i];
+      b += sa.part[kStartCap + i];
+    }
+    a = wave_sum(a);
+    b = wave_sum(b);
+    if (threadIdx.x == 0) {
+      const double du = sqrt(a);
+      const double r0 = sa.rel ? sqrt(b) : du;
+      const double thr = sa.rel ? sa.tol * r0 : sa.tol;
+      int code = 0;
+      if (!(du > thr))
+        code = 1;
+      else if (sa.cap < 1)
+        code = 3;
+      sdec[0] = (double)code;
+      if (blockIdx.x == 0 && blockIdx.y == 0) {
+        sa.scal[SC_NORM] = du;
+        sa.scal[SC_NORM0] = r0;
+        sa.gate[0] = code ? 1 : 0;
+        sa.gate[1] = 0;
+        sa.gate[2] = code;
+        sa.gate[4] = 0;
+        sa.gate[5] = 0;
+        sa.mb[SC_NORM] = du;
+        sa.mb[SC_NORM0] = r0;
+        sa.mb[MB_START] = du;
+        sa.mb[MB_USED] = 0.0;
+        sa.mb[MB_COUNT] = 0.0;
+        sa.mb[MB_CODE] = (double)code;
+        const int seq = sa.gate[3] + 1;
+        sa.gate[3] = seq;
+        __threadfence_system();
+        sa.mb[MB_SEQ] = (double)seq;
+      }
+    }
+  }
   const int H = 2 * nu + 2;
   Tile<T> t = make_tile<T>(smem, TX, TY, H, HAS_F);
   tile_load<T, HAS_F>(t, uin, f, cx, cz, N, K, P, zero_u != 0);
   __syncthreads();
+  if (SMODE == START_TEST && sdec[0] != 0.0) return;  // no V-cycle needed
+  if (SMODE == START_PARTIALS) {
+    // the start as loaded: residual norm and boundary-data norm partials
+    double v2[2] = {0.0, 0.0};
+    for (FlatWalk w(TX); w.r < TY; w.next()) {
+      const int lr = H + w.r, lc = H + w.c;
+      const int j = t.j0 + lr, k = t.k0 + lc;
+      if (j >= N || k < 1 || k > K - 2) continue;
+      const double r = (double)tile_residual<T, false>(t, lr, lc, j, N, DH2);
+      v2[0] += r * r;
+      if (k == 1 || k == K - 2) {
+        const int s = lr * t.LS + lc;
+        const T uW = (k - 1 == 0) ? t.su[s - 1] : T(0);
+        const T uE = (k + 1 == K - 1) ? t.su[s + 1] : T(0);
+        const T rb = T(0) - (t.scx[s] * uW + t.scx[s + 1] * uE) / DH2;
+        v2[1] += (double)(rb * rb);
+      }
+    }
+    block_sums<2>(v2, sdec);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+      const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+      sa.part[bid] = v2[0];
+      sa.part[kStartCap + bid] = v2[1];
+    }
+  }
   tile_sweeps<T, HAS_F, EXACT>(t, nu, N, K, DH2, relax);
   // residual on the interior plus one ring; it replaces f (read at the same
   // point only), or goes over scx (after a barrier, from registers), or to
@@ -379,18 +470,6 @@ __global__ __launch_bounds__(TILE_THREADS) void k_down(
 // Level-0 finishing modes of k_up.
 enum { UP_STORE = 0, UP_NORM = 1, UP_ADD64 = 2 };
 
-__device__ inline double block_sum(double v, double *sh) {
-  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-  const int n = blockDim.x * blockDim.y;
-  sh[tid] = v;
-  __syncthreads();
-  for (int s = n / 2; s > 0; s >>= 1) {
-    if (tid < s) sh[tid] += sh[tid + s];
-    __syncthreads();
-  }
-  return sh[0];
-}
-
 // Prolongation + correction + post-smoothing.
 template <typename T, bool HAS_F, bool EXACT, bool FULL_TOP, int MODE>
 __global__ __launch_bounds__(TILE_THREADS) void k_up(
@@ -426,8 +505,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_up(
     }
   }
   if (MODE == UP_NORM) {
-    __shared__ double red[TILE_THREADS];
-    const double s = block_sum(acc, red);
+    __shared__ double red[16];
+    const double s = block_sum1(acc, red);
     if (threadIdx.x == 0 && threadIdx.y == 0)
       partial[blockIdx.y * gridDim.x + blockIdx.x] = s;
   }
@@ -762,7 +841,7 @@ bool tiles_fit(const qdeq_ctx *c, int tx, int ty) {
 
 template <typename T, bool EXACT>
 static void launch_down(qdeq_ctx *c, LevelBufs<T> *B, int l, bool has_f,
-                        bool zero_u) {
+                        bool zero_u, int smode = DOWN_PLAIN) {
   const LevelDims &d = c->lv[l];
   const LevelDims &dc = c->lv[l + 1];
   const TileShape sh = tile_shape(c, l, true);
@@ -774,16 +853,46 @@ static void launch_down(qdeq_ctx *c, LevelBufs<T> *B, int l, bool has_f,
   const size_t lds = tile_lds_bytes(TX, TY, H, has_f, sizeof(T), true,
                                     TX_THREADS * sh.tyt);
   const T DH2 = (T)(d.DH * d.DH), relax = (T)c->cfg.relax;
-  if (has_f)
-    hipLaunchKernelGGL((k_down<T, true, EXACT>), grd, blk, lds,
-                       c->stream, B[l].u, B[l].r, B[l].f, B[l].cx, B[l].cz, d.N,
-                       d.K, d.P, DH2, relax, nu, zero_u ? 1 : 0, B[l + 1].f,
-                       dc.N, dc.K, dc.P, TX, TY, (const int *)c->gate_ptr);
+  StartArgs sa{};
+  if (smode != DOWN_PLAIN) {
+    const TileShape s0 = tile_shape(c, 0, true);
+    const LevelDims &d0 = c->lv[0];
+    sa.part = c->partial + kStartOff;
+    sa.nb = ((d0.K + s0.tx - 1) / s0.tx) * ((d0.N + s0.ty - 1) / s0.ty);
+    sa.scal = c->scal;
+    sa.mb = c->mbox_dev;
+    sa.gate = c->gate;
+    sa.rel = c->cfg.tol_mode == QDEQ_TOL_REL ? 1 : 0;
+    sa.cap = c->cfg.max_vcycles;
+    sa.tol = c->cfg.tol;
+  }
+#define DOWN_ARGS                                                                   \
+  grd, blk, lds, c->stream, B[l].u, B[l].r, B[l].f, B[l].cx, B[l].cz, d.N, d.K, d.P, \
+      DH2, relax, nu, zero_u ? 1 : 0, B[l + 1].f, dc.N, dc.K, dc.P, TX, TY,          \
+      (const int *)c->gate_ptr, sa
+  if (smode == START_PARTIALS)
+    hipLaunchKernelGGL((k_down<T, false, EXACT, START_PARTIALS>), DOWN_ARGS);
+  else if (smode == START_TEST)
+    hipLaunchKernelGGL((k_down<T, true, EXACT, START_TEST>), DOWN_ARGS);
+  else if (has_f)
+    hipLaunchKernelGGL((k_down<T, true, EXACT, DOWN_PLAIN>), DOWN_ARGS);
   else
-    hipLaunchKernelGGL((k_down<T, false, EXACT>), grd, blk, lds,
-                       c->stream, B[l].u, B[l].r, B[l].f, B[l].cx, B[l].cz, d.N,
-                       d.K, d.P, DH2, relax, nu, zero_u ? 1 : 0, B[l + 1].f,
-                       dc.N, dc.K, dc.P, TX, TY, (const int *)c->gate_ptr);
+    hipLaunchKernelGGL((k_down<T, false, EXACT, DOWN_PLAIN>), DOWN_ARGS);
+#undef DOWN_ARGS
+}
+
+// Whether the start of the next FP64 solve may be folded into its first
+// V-cycle's down kernels: two tile levels at least, and the start
+// partials of every level-0 tile and the up kernel's norm partials fit
+// their regions of c->partial.
+bool fused_start_fits(const qdeq_ctx *c) {
+  if (c->cfg.precision != QDEQ_PREC_FP64 || !use_tiled(c) || tail_entry_level(c) < 2)
+    return false;
+  const LevelDims &d = c->lv[0];
+  const TileShape dn = tile_shape(c, 0, true), up = tile_shape(c, 0, false);
+  const long nd = (long)((d.K + dn.tx - 1) / dn.tx) * ((d.N + dn.ty - 1) / dn.ty);
+  const long nu = (long)((d.K + up.tx - 1) / up.tx) * ((d.N + up.ty - 1) / up.ty);
+  return nd <= kStartCap && nu <= kStartOff;
 }
 
 template <typename T, bool EXACT, bool HAS_F, int MODE>
@@ -836,9 +945,14 @@ template <typename T, bool EXACT>
 static void vcycle_tiled(qdeq_ctx *c, LevelBufs<T> *B, bool level0_has_f,
                          double *partial, double *u64, int *nparts) {
   const int lt = tail_entry_level(c);
+  // the start of the solve folded into this cycle (FP64 only)
+  const bool fuse = c->fuse_start_next && !level0_has_f && lt >= 2;
+  c->fuse_start_next = false;
   for (int l = 0; l < lt; l++) {
     if (l == 1) timing_mark(c, QDEQ_TIME_DOWN1, 0);
-    launch_down<T, EXACT>(c, B, l, l > 0 || level0_has_f, l > 0 || level0_has_f);
+    const int smode = !fuse ? DOWN_PLAIN : (l == 0 ? START_PARTIALS
+                                                    : (l == 1 ? START_TEST : DOWN_PLAIN));
+    launch_down<T, EXACT>(c, B, l, l > 0 || level0_has_f, l > 0 || level0_has_f, smode);
     if (l == 1) timing_mark(c, QDEQ_TIME_DOWN1, 1);
   }
   timing_mark(c, QDEQ_TIME_TAIL, 0);
@@ -1096,10 +1210,12 @@ int tiled_setup(qdeq_ctx *c) {
 #define SETA(kern)                                                       \
   (void)hipFuncSetAttribute((const void *)(kern),                        \
                             hipFuncAttributeMaxDynamicSharedMemorySize, big)
-  SETA((k_down<double, true, true>));
-  SETA((k_down<double, false, true>));
-  SETA((k_down<float, true, false>));
-  SETA((k_down<float, false, false>));
+  SETA((k_down<double, true, true, DOWN_PLAIN>));
+  SETA((k_down<double, false, true, DOWN_PLAIN>));
+  SETA((k_down<double, false, true, START_PARTIALS>));
+  SETA((k_down<double, true, true, START_TEST>));
+  SETA((k_down<float, true, false, DOWN_PLAIN>));
+  SETA((k_down<float, false, false, DOWN_PLAIN>));
   SETA((k_up<double, true, true, true, UP_STORE>));
   SETA((k_up<double, false, true, true, UP_NORM>));
   SETA((k_up<float, true, false, true, UP_STORE>));

@@ -141,6 +141,9 @@ struct qdeq_ctx {
   // interior is then at plain_interior (same layout as u)
   bool cand_in_u = false;
   const double *plain_interior = nullptr;
+  // the next launched V-cycle carries the start of its solve (the start
+  // norms and test folded into its down kernels, see mg_tiled.hip)
+  bool fuse_start_next = false;
   std::string err;
   // RK43 host state (odeSolver.cpp:1338-1363)
   bool rk_ready = false;
@@ -260,6 +263,7 @@ void vcycle_tiled_fp64(qdeq_ctx *c, double *partial, int *nparts);
 void vcycle_tiled_fp32_ir(qdeq_ctx *c);
 int tiled_setup(qdeq_ctx *c);
 bool tiles_fit(const qdeq_ctx *c, int tx, int ty);
+bool fused_start_fits(const qdeq_ctx *c);
 int autotune_tiles(qdeq_ctx *c, int reps);
 int set_tile_shapes(qdeq_ctx *c, int nlevels, const int *shapes);
 int kernel_breakdown(qdeq_ctx *c, int reps, int *n, double *us);
@@ -310,6 +314,43 @@ __device__ inline void coarse_matvec(const float *__restrict__ minvT, const T *g
 }
 
 int build_coarse_inverse(qdeq_ctx *c);
+
+// Sum over the 64 lanes of a wavefront by a butterfly over lane distances
+// 32, 16, ..., 1: a fixed order of additions, and every lane ends with the
+// same bits (each pair adds the same two values).
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Sums of NV values over the workgroup, returned to every thread: wavefront
+// butterflies, then the wavefront sums added in wavefront order (one
+// barrier).  sh: LDS for NV doubles per wavefront, not reused by the caller
+// before a later barrier.
+template <int NV>
+__device__ inline void block_sums(double (&v)[NV], double *sh) {
+  const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  const int nw = (blockDim.x * blockDim.y * blockDim.z + 63) >> 6;
+#pragma unroll
+  for (int q = 0; q < NV; q++) v[q] = wave_sum(v[q]);
+  if ((tid & 63) == 0)
+#pragma unroll
+    for (int q = 0; q < NV; q++) sh[q * 16 + (tid >> 6)] = v[q];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    double a = sh[q * 16];
+    for (int w = 1; w < nw; w++) a = a + sh[q * 16 + w];
+    v[q] = a;
+  }
+}
+
+__device__ inline double block_sum1(double v, double *sh) {
+  double a[1] = {v};
+  block_sums<1>(a, sh);
+  return a[0];
+}
 
 // Entries of c->partial: the FP64 fast cycle's level-0 up kernel writes one
 // norm partial per tile, so a level-0 tiling may have at most this many
