@@ -571,9 +571,6 @@ def main():
               "process per GPU). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
-    if args.ensemble and "GPU_MAX_HW_QUEUES" not in os.environ:
-        # more hardware queues let the members' streams run concurrently
-        os.environ["GPU_MAX_HW_QUEUES"] = "16"
     from qdeqcycle_fdm_amd import config, lib
     lib.load()
     if args.mixed2049:

@@ -257,6 +257,10 @@ int qdeq_create(const qdeq_config *cfg, const double *mu,
   // (the path FP32-IR solves always take) instead of the device
   const char *hs = std::getenv("QDEQ_HOST_START");
   c->host_start = hs && hs[0] == '1';
+  // QDEQ_STAGE_GRAPHS=0: graph launches replay single V-cycles only (the
+  // stage's state, start and first batch are launched one by one)
+  const char *sg = std::getenv("QDEQ_STAGE_GRAPHS");
+  c->stage_graphs_off = sg && sg[0] == '0';
   *out = c;
   return QDEQ_OK;
 }
@@ -669,8 +673,15 @@ int qdeq_rk43_setup(qdeq_ctx *c, const double *a, const double *b,
   QDEQ_HIP(c, hipMemcpyAsync(c->stage_U, U0, nx * nz * sizeof(double),
                              hipMemcpyHostToDevice, c->stream));
   launch_pack(c, c->stage_U);
+  // the t = 0 stage reads t and dt on the device
+  const std::vector<double> tdt{0.0, c->cfg.ini_dt};
+  QDEQ_HIP(c, hipMemcpyAsync(c->scal + SC_T, tdt.data(), 2 * sizeof(double),
+                             hipMemcpyHostToDevice, c->stream));
   QDEQ_HIP(c, hipMemsetAsync(c->flags, 0, sizeof(int), c->stream));
   QDEQ_HIP(c, hipGetLastError());
+  // the copies read host vectors local to this call (and the caller's
+  // arrays): they must have completed before it returns
+  QDEQ_HIP(c, hipStreamSynchronize(c->stream));
   c->t = 0.0;
   c->dt = c->cfg.ini_dt;  // odeSolver.cpp:1363
   c->it = 0;
@@ -769,17 +780,26 @@ static int rk_advance(qdeq_ctx *c, bool block, int *done) {
       case RK_ATTEMPT_BEGIN:  // odeSolver.cpp:1466-1604
         r.attempts = r.attempts + 1;
         if (c->t + c->dt > p.tmax) c->dt = p.tmax - c->t;
-        launch_zero_stage_rates(c);
+        launch_zero_stage_rates(c, c->t, c->dt);
         r.s = 2;
         r.final_stage = false;
         r.phase = RK_STAGE_BEGIN;
         break;
       case RK_STAGE_BEGIN:  // boundary data, then the elliptic solve
-        if (!r.staged) launch_stage(c, r.s, c->t, c->dt);
+        c->solve_key = (r.final_stage ? 8 : 0) + r.s;
+        if (stage_graphs(c)) {
+          // the stage's state (or the previous stage's fault update with
+          // it), start and first V-cycles as one graph replay
+          const int kind = r.staged ? 1 : 0;
+          r.staged = false;
+          if ((st = mg_begin_stage(c, kind, r.s))) return st;
+          r.phase = RK_SOLVING;
+          break;
+        }
+        if (!r.staged) launch_stage(c, r.s);
         r.staged = false;
         if ((st = guess_before_solve(c))) return st;
         if ((st = proj_before_solve(c))) return st;
-        c->solve_key = (r.final_stage ? 8 : 0) + r.s;
         if ((st = mg_begin(c))) return st;
         r.phase = RK_SOLVING;
         break;
@@ -800,7 +820,9 @@ static int rk_advance(qdeq_ctx *c, bool block, int *done) {
             c->timed_calls += 1;
           }
         } else if (c->rk_ready && !r.final_stage && r.s < 6) {
-          launch_update_stage(c, r.s, c->t, c->dt);  // and stage s + 1
+          // the fault update and stage s + 1's state: one launch, or the
+          // first node of stage s + 1's graph
+          if (!stage_graphs(c)) launch_update_stage(c, r.s);
           r.staged = true;
         } else {
           launch_fault_update(c, r.s);
@@ -810,7 +832,7 @@ static int rk_advance(qdeq_ctx *c, bool block, int *done) {
             r.s++;
             r.phase = RK_STAGE_BEGIN;
           } else {
-            launch_combine(c, c->dt);
+            launch_combine(c);
             if ((st = rk_readback(c, true))) return st;
             r.phase = RK_WAIT_COMBINE;
           }
@@ -846,7 +868,7 @@ static int rk_advance(qdeq_ctx *c, bool block, int *done) {
         }
         c->attempts = r.attempts;
         c->t = c->t + c->dt;  // odeSolver.cpp:1605-1619
-        launch_commit(c);
+        launch_commit(c, c->t);
         r.s = 1;
         r.final_stage = true;
         r.phase = RK_STAGE_BEGIN;

@@ -192,14 +192,16 @@ __device__ inline void stage_node(int k, int s, double dt, double t, double rati
   row[K - 1] = 0.50 * Vini * (t + ratio * dt) + ss[k];
 }
 
-__global__ void k_stage(int s, double dt, double t, double ratio, double Vini,
+// tdt: t and dt of the attempt on the device
+__global__ void k_stage(int s, const double *tdt, double ratio, double Vini,
                         const double *slip, const double *psi,
                         const double *ss, double *const *FV,
                         double *const *Fd, double *Ys, double *Yp, double *u,
                         int nk, int K, int P) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
   if (k > nk) return;
-  stage_node(k, s, dt, t, ratio, Vini, slip, psi, ss, FV, Fd, Ys, Yp, u, K, P);
+  stage_node(k, s, tdt[1], tdt[0], ratio, Vini, slip, psi, ss, FV, Fd, Ys, Yp, u, K,
+             P);
 }
 
 // Fault update after a stage solve: Ts from the compact u (columns 0..3 are
@@ -253,7 +255,8 @@ struct UpdateArgs {
 };
 struct StageArgs {
   int s;
-  double dt, t, ratio, Vini;
+  const double *tdt;  // t, dt of the attempt
+  double ratio, Vini;
   const double *slip, *psi, *ss;
   double *const *FV;
   double *const *Fd;
@@ -267,12 +270,17 @@ __global__ void k_update_stage(double *u, int nk, int K, int P, UpdateArgs ua,
   update_node(k, u, P, ua.mu_f, ua.tau_n0, ua.Yp, ua.a, ua.b, ua.eta, ua.V0, ua.f0,
               ua.L, ua.c0, ua.c1, ua.c2, ua.c3, ua.Ts, ua.Tn, ua.V, ua.dpsi,
               ua.flags);
-  stage_node(k, sa.s, sa.dt, sa.t, sa.ratio, sa.Vini, sa.slip, sa.psi, sa.ss, sa.FV,
-             sa.Fd, sa.Ys, sa.Yp, u, K, P);
+  stage_node(k, sa.s, sa.tdt[1], sa.tdt[0], sa.ratio, sa.Vini, sa.slip, sa.psi, sa.ss,
+             sa.FV, sa.Fd, sa.Ys, sa.Yp, u, K, P);
 }
 
-__global__ void k_zero_rates(double *const *FV, double *const *Fd, int nz) {
+__global__ void k_zero_rates(double *const *FV, double *const *Fd, int nz,
+                             double *tdt, double t, double dt) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k == 0) {
+    tdt[0] = t;
+    tdt[1] = dt;
+  }
   if (k >= nz) return;
   for (int s = 2; s <= 6; s++) {
     FV[s][k] = 0.0;
@@ -285,11 +293,12 @@ __global__ void k_zero_rates(double *const *FV, double *const *Fd, int nz) {
 // squared differences are summed by one lane in node order, so totErr has
 // the reference's summation order.
 __global__ __launch_bounds__(1024) void k_combine(
-    double dt, const double *slip, const double *psi, double *const *FV,
+    const double *tdt, const double *slip, const double *psi, double *const *FV,
     double *const *Fd, double *y3s, double *y3p, double *y4s, double *y4p,
     int nk, double *out) {
   __shared__ double s1[1024], s2[1024];
   const int tid = threadIdx.x;
+  const double dt = tdt[1];
   double e1 = 0.0, e2 = 0.0;  // meaningful in lane 0 only
   for (int base = 1; base <= nk; base += 1024) {
     const int k = base + tid;
@@ -335,10 +344,13 @@ __global__ __launch_bounds__(1024) void k_combine(
   }
 }
 
-// Accept: y_n <- y4, F_1 <- 0 (odeSolver.cpp:1606-1614).
+// Accept: y_n <- y4, F_1 <- 0 (odeSolver.cpp:1606-1614); the device time
+// becomes t_new = t + dt.
 __global__ void k_commit(double *slip, double *psi, const double *y4s,
-                         const double *y4p, double *V1, double *d1, int nk) {
+                         const double *y4p, double *V1, double *d1, int nk,
+                         double *tdt, double t_new) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
+  if (k == 1) tdt[0] = t_new;
   if (k > nk) return;
   slip[k] = y4s[k];
   psi[k] = y4p[k];
@@ -434,10 +446,10 @@ static double *const *dev_FV(qdeq_ctx *c) {
 }
 static double *const *dev_Fd(qdeq_ctx *c) { return dev_FV(c) + 7; }
 
-void launch_stage(qdeq_ctx *c, int s, double t, double dt) {
+void launch_stage(qdeq_ctx *c, int s) {
   FaultBufs &f = c->fb;
   hipLaunchKernelGGL(k_stage, nodes_grid(c->cfg.nk), dim3(FT), 0, c->stream, s,
-                     dt, t, kC[s], c->cfg.Vini, f.slip, f.psi, f.slip_shift,
+                     c->scal + SC_T, kC[s], c->cfg.Vini, f.slip, f.psi, f.slip_shift,
                      dev_FV(c), dev_Fd(c), f.Ys[s], f.Yp[s], c->d[0].u,
                      c->cfg.nk, c->lv[0].K, c->lv[0].P);
 }
@@ -451,34 +463,35 @@ void launch_fault_update(qdeq_ctx *c, int s) {
                      c->c_stress[3], f.Ts, f.Tn, f.FV[s], f.Fd[s], c->flags);
 }
 
-void launch_update_stage(qdeq_ctx *c, int s, double t, double dt) {
+void launch_update_stage(qdeq_ctx *c, int s) {
   FaultBufs &f = c->fb;
   const UpdateArgs ua{f.mu_f, f.tau_n0, f.Yp[s], f.a, f.b, f.eta, c->cfg.V0,
                       c->cfg.f0, c->cfg.L, c->c_stress[0], c->c_stress[1],
                       c->c_stress[2], c->c_stress[3], f.Ts, f.Tn, f.FV[s], f.Fd[s],
                       c->flags};
-  const StageArgs sa{s + 1, dt, t, kC[s + 1], c->cfg.Vini, f.slip, f.psi,
+  const StageArgs sa{s + 1, c->scal + SC_T, kC[s + 1], c->cfg.Vini, f.slip, f.psi,
                      f.slip_shift, dev_FV(c), dev_Fd(c), f.Ys[s + 1], f.Yp[s + 1]};
   hipLaunchKernelGGL(k_update_stage, nodes_grid(c->cfg.nk), dim3(FT), 0, c->stream,
                      c->d[0].u, c->cfg.nk, c->lv[0].K, c->lv[0].P, ua, sa);
 }
 
-void launch_zero_stage_rates(qdeq_ctx *c) {
+void launch_zero_stage_rates(qdeq_ctx *c, double t, double dt) {
   hipLaunchKernelGGL(k_zero_rates, nodes_grid(c->nz), dim3(FT), 0, c->stream,
-                     dev_FV(c), dev_Fd(c), c->nz);
+                     dev_FV(c), dev_Fd(c), c->nz, c->scal + SC_T, t, dt);
 }
 
-void launch_combine(qdeq_ctx *c, double dt) {
+void launch_combine(qdeq_ctx *c) {
   FaultBufs &f = c->fb;
-  hipLaunchKernelGGL(k_combine, dim3(1), dim3(1024), 0, c->stream, dt, f.slip,
+  hipLaunchKernelGGL(k_combine, dim3(1), dim3(1024), 0, c->stream, c->scal + SC_T, f.slip,
                      f.psi, dev_FV(c), dev_Fd(c), f.y3s, f.y3p, f.y4s, f.y4p,
                      c->cfg.nk, c->scal + SC_TOTERR);
 }
 
-void launch_commit(qdeq_ctx *c) {
+void launch_commit(qdeq_ctx *c, double t_new) {
   FaultBufs &f = c->fb;
   hipLaunchKernelGGL(k_commit, nodes_grid(c->cfg.nk), dim3(FT), 0, c->stream,
-                     f.slip, f.psi, f.y4s, f.y4p, f.FV[1], f.Fd[1], c->cfg.nk);
+                     f.slip, f.psi, f.y4s, f.y4p, f.FV[1], f.Fd[1], c->cfg.nk,
+                     c->scal + SC_T, t_new);
 }
 
 void launch_maxima(qdeq_ctx *c) {

@@ -758,6 +758,8 @@ int qdeq_guess_train(qdeq_guess *g, const qdeq_train_config *tc,
     std::shuffle(valid.begin(), valid.end(), rng);
     GHIP(g, hipMemcpyAsync(g->idx, valid.data(), valid.size() * sizeof(int),
                            hipMemcpyHostToDevice, g->stream));
+    // the next epoch reshuffles the host list: this copy must be done first
+    GHIP(g, hipStreamSynchronize(g->stream));
     const int nbl = std::min(nb, (1 << 16));
     for (int b = 0; b < nbl; b++, step++) {
       nn::gather(g->stream, g->xs, g->ts, g->idx + (size_t)b * B, B, Mb, g->nk,

@@ -9,6 +9,7 @@
 // Sums are formed in the reference's order and the build uses
 // -ffp-contract=off, so the FP64 reference cycle reproduces the reference's
 // iterates bit for bit.
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -611,11 +612,17 @@ static void launch_bdres(qdeq_ctx *c) {
                      c->mbox_dev + SC_NORM0);
 }
 
-// Whether solves of this context decide their start on the device: FP64
-// tiled solves, unless QDEQ_HOST_START=1.  The choices and norms are the
-// same either way; the host then waits only for the V-cycles.
-bool device_start(const qdeq_ctx *c) {
-  return !c->host_start && c->cfg.precision == QDEQ_PREC_FP64 && use_tiled(c);
+// Whether solves of this context decide their start on the device: tiled
+// solves (FP64, and FP32-IR without a candidate start), unless
+// QDEQ_HOST_START=1.  The choices and norms are the same either way; the
+// host then waits only for the V-cycles.
+bool device_start(const qdeq_ctx *c) { return !c->host_start && use_tiled(c); }
+
+// The same for a solve with a candidate start (projection or learned
+// guess): FP64 only; FP32-IR candidates always take the host path, which
+// expects the candidate in r and its norm in the mailbox.
+bool device_start_cand(const qdeq_ctx *c) {
+  return device_start(c) && c->cfg.precision != QDEQ_PREC_FP32_IR;
 }
 
 // Whether this context runs the fast cycle with the tiled kernels.
@@ -653,6 +660,8 @@ void mg_destroy_graphs(qdeq_ctx *c) {
   if (c->g_cycle) (void)hipGraphExecDestroy(c->g_cycle);
   c->g_cycle = nullptr;
   c->g_cycle_mode = -1;
+  for (auto &kv : c->g_stage) (void)hipGraphExecDestroy(kv.second);
+  c->g_stage.clear();
 }
 
 static int ensure_graph(qdeq_ctx *c) {
@@ -760,6 +769,46 @@ static int mbox_check(qdeq_ctx *c, int slot, bool block, int *ready) {
 // whether it is polled or waited for.
 static int launch_batch(qdeq_ctx *c, int k);
 
+// The device start of a solve without a candidate: FP64 folds it into the
+// first V-cycle's down kernels when that fits (fuse), else runs the start
+// partials and gate; FP32-IR runs the host path's residual and norm kernels
+// (the FP64 residual kept as the FP32 right-hand side) and the gate.  One
+// step of the mailbox sequence.
+static void launch_plain_dev_start(qdeq_ctx *c, bool fuse) {
+  const LevelDims &d = c->lv[0];
+  const int rel = c->cfg.tol_mode == QDEQ_TOL_REL ? 1 : 0;
+  if (c->cfg.precision == QDEQ_PREC_FP32_IR) {
+    if (rel)
+      hipLaunchKernelGGL(k_bdres_partial, dim3(BDRES_BLOCKS), dim3(NORM_THREADS), 0,
+                         c->stream, c->d[0].u, c->d[0].cx, d.N, d.K, d.P, d.DH * d.DH,
+                         c->partial + NORM_BLOCKS);
+    hipLaunchKernelGGL((k_resnorm_partial<true>), dim3(norm_blocks(c)),
+                       dim3(NORM_THREADS), 0, c->stream, c->d[0].u, c->d[0].cx,
+                       c->d[0].cz, d.N, d.K, d.P, d.DH * d.DH, c->partial, c->s[0].f,
+                       c->s[0].u, (const int *)nullptr);
+  } else if (fuse) {
+    c->fuse_start_next = true;
+    return;
+  } else {
+    hipLaunchKernelGGL(k_start_partials<false>, dim3(norm_blocks(c)), dim3(NORM_THREADS),
+                       0, c->stream, c->d[0].u, c->d[0].u, (const double *)nullptr,
+                       c->d[0].cx, c->d[0].cz, d.N, d.K, d.P, d.DH * d.DH, rel,
+                       c->partial);
+  }
+  hipLaunchKernelGGL(k_start_gate, dim3(1), dim3(NORM_THREADS), 0, c->stream,
+                     c->partial, norm_blocks(c), 0, rel, c->cfg.tol, c->cfg.max_vcycles,
+                     c->scal, c->mbox_dev, c->gate);
+}
+
+// V-cycles of the first batch of a solve: what the same RK43 stage needed in
+// the previous step (c->last_it_key), else what the last solve needed.
+static int first_batch(const qdeq_ctx *c) {
+  const int key = c->solve_key;
+  const int pred = (key >= 0 && c->last_it_key[key] >= 0) ? c->last_it_key[key]
+                                                           : c->last_it;
+  return std::min(std::max(pred, 1), 6);
+}
+
 int mg_begin(qdeq_ctx *c) {
   int st = ensure_graph(c);
   if (st != QDEQ_OK) return st;
@@ -773,8 +822,19 @@ int mg_begin(qdeq_ctx *c) {
   m.it = 0;
   m.sample = false;
   m.launched = 0;
-  m.dev_start = device_start(c);
+  m.dev_start = m.cand ? device_start_cand(c) : device_start(c);
   m.start_read = false;
+  if (m.dev_start && m.ir) {
+    // FP32-IR: the start decided on the device by the host path's kernels
+    // and rule (same sums, same test)
+    launch_plain_dev_start(c, false);
+    c->seq_issued += 1;
+    QDEQ_HIP(c, hipGetLastError());
+    int st = launch_batch(c, first_batch(c));
+    if (st != QDEQ_OK) return st;
+    m.phase = 2;
+    return QDEQ_OK;
+  }
   if (m.dev_start) {
     // norms, start choice and stop test on the device; the first batch is
     // launched at once, sized from the last solve
@@ -789,21 +849,16 @@ int mg_begin(qdeq_ctx *c) {
     const double *cand = !m.cand ? nullptr : (in_u ? c->d[0].u : c->d[0].r);
     // a plain start launched kernel by kernel: its norms and test ride on
     // the first V-cycle's down kernels (the same decisions, no start launches)
-    const bool fused = !cand && launch_direct(c) && fused_start_fits(c);
-    if (fused)
-      c->fuse_start_next = true;
-    else if (cand)
+    if (!cand) {
+      launch_plain_dev_start(c, launch_direct(c) && fused_start_fits(c));
+    } else {
       hipLaunchKernelGGL(k_start_partials<true>, dim3(norm_blocks(c)), dim3(NORM_THREADS), 0,
                          c->stream, plain, c->d[0].u, cand, c->d[0].cx, c->d[0].cz,
                          d.N, d.K, d.P, d.DH * d.DH, rel, c->partial);
-    else
-      hipLaunchKernelGGL(k_start_partials<false>, dim3(norm_blocks(c)), dim3(NORM_THREADS), 0,
-                         c->stream, plain, c->d[0].u, cand, c->d[0].cx, c->d[0].cz,
-                         d.N, d.K, d.P, d.DH * d.DH, rel, c->partial);
-    if (!fused)
       hipLaunchKernelGGL(k_start_gate, dim3(1), dim3(NORM_THREADS), 0, c->stream,
-                         c->partial, norm_blocks(c), m.cand ? 1 : 0, rel, c->cfg.tol,
+                         c->partial, norm_blocks(c), 1, rel, c->cfg.tol,
                          c->cfg.max_vcycles, c->scal, c->mbox_dev, c->gate);
+    }
     if (in_u) {
       // the plain start back over a candidate that was not chosen
       hipLaunchKernelGGL(k_restore_if, dim3(std::min(d.N, 1024)), dim3(256), 0,
@@ -817,10 +872,7 @@ int mg_begin(qdeq_ctx *c) {
     }
     c->seq_issued += 1;
     QDEQ_HIP(c, hipGetLastError());
-    const int key = c->solve_key;
-    const int pred = (key >= 0 && c->last_it_key[key] >= 0) ? c->last_it_key[key]
-                                                             : c->last_it;
-    int st = launch_batch(c, std::min(std::max(pred, 1), 6));
+    int st = launch_batch(c, first_batch(c));
     if (st != QDEQ_OK) return st;
     m.phase = 2;
     return QDEQ_OK;
@@ -831,6 +883,67 @@ int mg_begin(qdeq_ctx *c) {
   launch_resnorm(c, m.ir);
   QDEQ_HIP(c, hipGetLastError());
   m.phase = 1;
+  return QDEQ_OK;
+}
+
+// Whole RK43 stages as graph replays (where V-cycles are replays anyway):
+// the stage's state kernel (kind 0: k_stage of stage s; kind 1: the fault
+// update of stage s - 1 with the state of stage s), the plain device start
+// and the first batch of V-cycles are one captured graph per (kind, stage,
+// batch size), replayed for every solve of that stage; the same kernels with
+// the same arguments as the direct launches (t and dt are read on the
+// device), so the results do not change.  Later batches of the solve replay
+// the one-V-cycle graph.
+bool stage_graphs(const qdeq_ctx *c) {
+  return !c->stage_graphs_off && !launch_direct(c) && device_start(c) && c->timing == 0 &&
+         c->proj_slots == 0 &&
+         (c->guess == nullptr || c->guess_mode == QDEQ_GUESS_OFF);
+}
+
+int mg_begin_stage(qdeq_ctx *c, int kind, int s) {
+  int st = ensure_graph(c);
+  if (st != QDEQ_OK) return st;
+  MgSolve &m = c->mg;
+  m.ir = c->cfg.precision == QDEQ_PREC_FP32_IR;
+  m.cand = false;
+  c->cand_pending = false;
+  m.it = 0;
+  m.sample = false;
+  m.dev_start = true;
+  m.start_read = false;
+  const int B = first_batch(c);
+  const int gk = (kind * 8 + s) * 8 + B;
+  auto it = c->g_stage.find(gk);
+  hipGraphExec_t ex = nullptr;
+  if (it != c->g_stage.end()) {
+    ex = it->second;
+  } else {
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      if (kind == 1)
+        launch_update_stage(c, s - 1);
+      else
+        launch_stage(c, s);
+      launch_plain_dev_start(c, !m.ir && fused_start_fits(c));
+      c->gate_ptr = c->gate;
+      for (int i = 0; i < B; i++) launch_iteration(c);
+      c->gate_ptr = nullptr;
+      c->fuse_start_next = false;
+      e = hipStreamEndCapture(c->stream, &g);
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    if (e != hipSuccess)
+      return set_error(c, QDEQ_ERR_HIP,
+                       std::string("stage graph capture: ") + hipGetErrorString(e));
+    c->g_stage[gk] = ex;
+  }
+  QDEQ_HIP(c, hipGraphLaunch(ex, c->stream));
+  c->seq_issued += 1 + B;
+  m.launched = B;
+  m.seq_target = c->seq_issued;
+  m.phase = 2;
   return QDEQ_OK;
 }
 

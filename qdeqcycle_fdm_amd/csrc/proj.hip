@@ -326,7 +326,8 @@ int proj_before_solve(qdeq_ctx *c) {
   if (c->proj_slots < 1 || c->proj_count < 1) return QDEQ_OK;
   const LevelDims &d = c->lv[0];
   const ProjSlots sl = proj_order(c);
-  const bool dev = device_start(c);  // the start's norms come with the solve
+  // the start's norms come with the solve (the same test as mg_begin's)
+  const bool dev = device_start_cand(c);
   if (!dev) mbox_arm(c, SC_NORM_PRED);
   if (sl.cnt >= 2)
     hipLaunchKernelGGL(coef_kernel(sl.cnt), dim3(1), dim3(PROJ_THREADS),

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -59,6 +60,8 @@ enum {
   SC_VMAX = 3,
   SC_TSMAX = 4,
   SC_NORM_PRED = 5,  // residual norm of the candidate start
+  SC_T = 6,          // RK43 time t_n of the current attempt (the stage kernels
+  SC_DT = 7,         //   read t and dt here, so captured stages replay)
   MB_SEQ = 8,        // mailbox only: sequence number of the last V-cycle seen
   MB_COUNT = 9,      //   V-cycles done in the current solve
   MB_CODE = 10,      //   outcome of the last V-cycle (k_norm_gate)
@@ -116,6 +119,9 @@ struct qdeq_ctx {
   hipStream_t stream = nullptr;
   hipGraphExec_t g_cycle = nullptr;   // one V-cycle (or IR sweep) + residual norm
   int g_cycle_mode = -1;
+  // whole RK43 stages (state kernel, start, first V-cycle batch) by
+  // (kind, stage, batch size), see mg_begin_stage
+  std::map<int, hipGraphExec_t> g_stage;
   // V-cycles (and the learned guess's inference chain) are launched kernel by
   // kernel or as captured graphs.  Measured at 1025^2: one simulation steps
   // 3% faster with direct launches (a graph launch costs the GPU more than
@@ -128,6 +134,7 @@ struct qdeq_ctx {
   int launch_env = 0;
   bool in_ensemble = false;
   bool host_start = false;     // start decisions on the host (see mg_begin)
+  bool stage_graphs_off = false;  // QDEQ_STAGE_GRAPHS=0 (see stage_graphs)
   int last_it = 2;             // V-cycles of the last solve (first batch size)
   // RK43 solves: the stage's key (stage s, +8 for the solve after an
   // accepted step) while its solve runs, else -1, and the V-cycles of the
@@ -225,11 +232,17 @@ int set_error(qdeq_ctx *c, int code, const std::string &msg);
 // multigrid (mg.hip)
 int mg_solve(qdeq_ctx *c, int *vcycles, double *res0, double *res);
 int mg_begin(qdeq_ctx *c);
+// stage graphs: whether this context replays whole RK stages, and the
+// start of a stage solve as one graph replay (kind 0: stage s's state;
+// kind 1: stage s - 1's fault update and stage s's state)
+bool stage_graphs(const qdeq_ctx *c);
+int mg_begin_stage(qdeq_ctx *c, int kind, int s);
 int mg_poll(qdeq_ctx *c, bool block, int *done);
 int mg_build_coefficients(qdeq_ctx *c);
 void mg_destroy_graphs(qdeq_ctx *c);
 bool use_tiled(const qdeq_ctx *c);
 bool device_start(const qdeq_ctx *c);
+bool device_start_cand(const qdeq_ctx *c);
 
 void launch_resnorm_of(qdeq_ctx *c, const double *u, int off, int slot);
 void mbox_arm(qdeq_ctx *c, int slot);
@@ -393,13 +406,16 @@ void launch_nr(qdeq_ctx *c, const double *Ts, const double *Tn,
                double *V);
 void launch_dpsi(qdeq_ctx *c, const double *psi, const double *V,
                  const double *b, double *dpsi);
-void launch_stage(qdeq_ctx *c, int s, double t, double dt);
+// stage kernels: t and dt from scal[SC_T], scal[SC_DT]
+void launch_stage(qdeq_ctx *c, int s);
 // fault update of stage s and the stage state of s + 1 in one launch
-void launch_update_stage(qdeq_ctx *c, int s, double t, double dt);
+void launch_update_stage(qdeq_ctx *c, int s);
 void launch_fault_update(qdeq_ctx *c, int s);
-void launch_zero_stage_rates(qdeq_ctx *c);
-void launch_combine(qdeq_ctx *c, double dt);
-void launch_commit(qdeq_ctx *c);
+// zeroes the stage rates of a new attempt and sets its t, dt on the device
+void launch_zero_stage_rates(qdeq_ctx *c, double t, double dt);
+void launch_combine(qdeq_ctx *c);
+// accepts y4 and sets the device time to t_new
+void launch_commit(qdeq_ctx *c, double t_new);
 void launch_maxima(qdeq_ctx *c);
 
 }  // namespace qdeq

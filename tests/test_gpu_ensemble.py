@@ -1,7 +1,7 @@
 """GPU checks of ensemble stepping (qdeq_ensemble_step): members with uneven
 grids and seeded frictional parameters, stepped together from one host
 thread on their own streams, are each bit-identical to the same simulation
-run alone (FP64, FP32-IR, and FP64 with the projection start); a member
+run alone (FP64, FP32-IR, and both with the projection start); a member
 that fails (forced from the host: a V-cycle cap of one at an unreachable
 tolerance) reports its status without disturbing the others."""
 import numpy as np
@@ -36,7 +36,7 @@ def _engine(pb, precision=0, slots=0, **kw):
 
 
 @pytest.mark.parametrize("precision,slots", [(config.PREC_FP64, 0), (config.PREC_FP32_IR, 0),
-                                             (config.PREC_FP64, 6)])
+                                             (config.PREC_FP64, 6), (config.PREC_FP32_IR, 6)])
 def test_ensemble_members_equal_solo_runs(precision, slots):
     pbs = _members(6, seed=3)
     steps = 40

@@ -361,3 +361,30 @@ def test_rk43_production_cycle_matches_oracle():
     ar = sim.arrays()
     for key in ("slip", "psi", "V", "U"):
         np.testing.assert_array_equal(st[key], ar[key], err_msg=key)
+
+
+@pytest.mark.parametrize("n,levels", [(129, 4), (257, 5)])
+def test_ir_device_start_equals_host_start(n, levels, monkeypatch):
+    """FP32-IR solves decide their start on the device (the host path's
+    residual and norm kernels, the sums in its order, its test) and launch
+    their first V-cycles without waiting; QDEQ_HOST_START=1 keeps the host
+    decision.  The runs are bit for bit equal, V-cycle counts included."""
+    pb = config.make_bp1_problem(n, basin=True, levels=levels,
+                                 tol_mode=config.TOL_REL, tol=1e-10)
+    me = oracle.mu_equi(pb.mu)
+    runs = []
+    for host in (False, True):
+        if host:
+            monkeypatch.setenv("QDEQ_HOST_START", "1")
+        else:
+            monkeypatch.delenv("QDEQ_HOST_START", raising=False)
+        with lib.Engine(pb, me, **{**PROD, "precision": config.PREC_FP32_IR}) as eng:
+            eng.set_tiles(True, 64, 32)
+            eng.rk43_setup()
+            eng.rk43_step(60)
+            runs.append(eng.state())
+    a, b = runs
+    for key in ("t", "vcycles", "solves", "ir_sweeps"):
+        assert a["info"][key] == b["info"][key], key
+    for k in ("slip", "psi", "V", "U"):
+        np.testing.assert_array_equal(a[k], b[k])

@@ -115,19 +115,23 @@ def test_projection_and_learned_guess_are_exclusive():
         g.close()
 
 
-@pytest.mark.parametrize("slots", [0, 6])
-def test_device_start_equals_host_start(slots, monkeypatch):
-    """FP64 solves decide their start on the device (norms, candidate
-    choice, stop test) and launch the first V-cycles without waiting;
-    QDEQ_HOST_START=1 restores the host decision.  Same norms, same choices:
-    the runs are bit for bit equal."""
+@pytest.mark.parametrize("slots,precision", [(0, config.PREC_FP64), (6, config.PREC_FP64),
+                                             (0, config.PREC_FP32_IR),
+                                             (6, config.PREC_FP32_IR)])
+def test_device_start_equals_host_start(slots, precision, monkeypatch):
+    """Solves decide their start on the device (norms, candidate choice,
+    stop test) and launch the first V-cycles without waiting; QDEQ_HOST_START=1
+    restores the host decision.  Same norms, same choices: the runs are bit
+    for bit equal.  FP32-IR solves with a projection start take the host
+    path either way (the projection then puts its start where that path
+    reads it); the starts are used."""
     runs = []
     for host in (False, True):
         if host:
             monkeypatch.setenv("QDEQ_HOST_START", "1")
         else:
             monkeypatch.delenv("QDEQ_HOST_START", raising=False)
-        with _engine(129, tol=1e-10) as eng:
+        with _engine(129, tol=1e-10, precision=precision) as eng:
             if slots:
                 eng.projection_enable(slots)
             eng.rk43_setup()
@@ -137,7 +141,10 @@ def test_device_start_equals_host_start(slots, monkeypatch):
     assert a["info"]["vcycles"] == b["info"]["vcycles"]
     assert a["info"]["solves"] == b["info"]["solves"]
     assert pa == pb_
+    if slots:
+        assert pa["used"] > 0.5 * pa["offered"] > 0
     for k in ("slip", "psi", "V", "U"):
+        assert np.all(np.isfinite(a[k])), k
         np.testing.assert_array_equal(a[k], b[k])
 
 
