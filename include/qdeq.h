@@ -300,6 +300,15 @@ int qdeq_projection_inject(qdeq_ctx *ctx, double factor);
 int qdeq_projection_stats(qdeq_ctx *ctx, long long *offered,
                           long long *used, double *fit_ratio, int *columns);
 
+/* Solves launched ahead by qdeq_rk43_step (no reference counterpart): with
+ * one simulation launching kernel by kernel, each stage's fault update,
+ * start and first V-cycles are queued behind the previous stage's solve and
+ * run only if that solve converged with the V-cycles already queued; else
+ * they abandon themselves and the solve goes on.  Results are the same
+ * either way (QDEQ_SPECULATE=0 in the environment at qdeq_create turns it
+ * off).  Counts of such solves taken over and abandoned. */
+int qdeq_launch_ahead_stats(qdeq_ctx *ctx, long long *taken, long long *abandoned);
+
 /* Copies the current state to host arrays (any may be NULL): U (nx*nz, with
  * ghosts as ode_solve leaves them), slip, psi, V, dpsi, Ts (nz). */
 int qdeq_rk43_get_state(qdeq_ctx *ctx, qdeq_step_info *info, double *U,

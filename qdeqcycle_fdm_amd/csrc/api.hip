@@ -199,10 +199,10 @@ int qdeq_create(const qdeq_config *cfg, const double *mu,
   CREATE_HIP(alloc_zero(&c->scal, SC_COUNT, c->stream));
   CREATE_HIP(alloc_zero(&c->flags, 4, c->stream));
   CREATE_HIP(hipHostMalloc(&c->scal_host, SC_COUNT * sizeof(double)));
-  CREATE_HIP(hipHostMalloc(&c->mbox_host, SC_COUNT * sizeof(double),
+  CREATE_HIP(hipHostMalloc(&c->mbox_host, 2 * SC_COUNT * sizeof(double),
                            hipHostMallocMapped | hipHostMallocCoherent));
   CREATE_HIP(hipHostGetDevicePointer((void **)&c->mbox_dev, c->mbox_host, 0));
-  std::memset(c->mbox_host, 0, SC_COUNT * sizeof(double));
+  std::memset(c->mbox_host, 0, 2 * SC_COUNT * sizeof(double));
   CREATE_HIP(alloc_zero(&c->gate, 8, c->stream));
   CREATE_HIP(hipHostMalloc(&c->flags_host, 4 * sizeof(int)));
   CREATE_HIP(alloc_zero(&c->stage_U, (size_t)2 * c->nx * c->nz, c->stream));
@@ -261,6 +261,9 @@ int qdeq_create(const qdeq_config *cfg, const double *mu,
   // stage's state, start and first batch are launched one by one)
   const char *sg = std::getenv("QDEQ_STAGE_GRAPHS");
   c->stage_graphs_off = sg && sg[0] == '0';
+  // QDEQ_SPECULATE=0: every stage waits for the previous solve's end
+  const char *sp = std::getenv("QDEQ_SPECULATE");
+  c->spec_off = sp && sp[0] == '0';
   *out = c;
   return QDEQ_OK;
 }
@@ -689,9 +692,13 @@ int qdeq_rk43_setup(qdeq_ctx *c, const double *a, const double *b,
   c->errA[0] = c->errA[1] = 0.0;
   c->totErr = 0.0;
   c->vcycles = c->solves = c->ir_sweeps = 0;
+  c->spec_taken = c->spec_abandoned = 0;
   c->rk_ready = false;
   c->rk.phase = 0;  // idle
   c->rk.staged = false;
+  c->rk.spec = false;
+  c->spec_pending = false;
+  c->mb_page = 0;
   c->mg.phase = 0;
   guess_reset(c);
   proj_reset(c);
@@ -752,6 +759,18 @@ static int rk_readback_ready(qdeq_ctx *c, bool block, int *ready) {
   return QDEQ_OK;
 }
 
+// Whether the next stage may be launched ahead of the current solve's end:
+// a single simulation launching kernel by kernel (FP64, start decided on the
+// device, no candidate start, no fault-update timing), with the current
+// solve's V-cycles in flight and nothing else launched ahead.
+// QDEQ_SPECULATE=0 turns it off.
+static bool spec_allowed(const qdeq_ctx *c) {
+  return !c->spec_off && launch_direct(c) && !stage_graphs(c) && device_start(c) &&
+         c->cfg.precision == QDEQ_PREC_FP64 && c->timing != QDEQ_TIME_FAULT &&
+         c->proj_slots == 0 && (c->guess == nullptr || c->guess_mode == QDEQ_GUESS_OFF) &&
+         c->mg.phase == 2 && c->mg.dev_start && !c->mg.cand && !c->spec_pending;
+}
+
 // Advances the machine as far as it can; *done = 1 when the requested steps
 // (and the final flag check) are complete.  With block, every wait blocks.
 static int rk_advance(qdeq_ctx *c, bool block, int *done) {
@@ -787,6 +806,19 @@ static int rk_advance(qdeq_ctx *c, bool block, int *done) {
         break;
       case RK_STAGE_BEGIN:  // boundary data, then the elliptic solve
         c->solve_key = (r.final_stage ? 8 : 0) + r.s;
+        if (r.spec) {
+          // launched ahead behind the previous solve, which converged with
+          // the work already queued: take it over
+          c->mg = c->mg_next;
+          c->mb_page = c->mg.page;
+          c->spec_pending = false;
+          c->spec_taken++;
+          r.spec = false;
+          r.staged = false;
+          r.phase = RK_SOLVING;
+          break;
+        }
+        c->mb_page = 0;
         if (stage_graphs(c)) {
           // the stage's state (or the previous stage's fault update with
           // it), start and first V-cycles as one graph replay
@@ -804,8 +836,30 @@ static int rk_advance(qdeq_ctx *c, bool block, int *done) {
         r.phase = RK_SOLVING;
         break;
       case RK_SOLVING: {
+        if (!r.spec && c->rk_ready && !r.final_stage && r.s < 6 && spec_allowed(c)) {
+          // stage s + 1 queued behind this solve: the fault update of stage
+          // s with stage s + 1's state, its start and first batch, all
+          // gated on this solve's outcome
+          launch_update_stage(c, r.s, c->gate);
+          const int key = c->solve_key;
+          c->solve_key = r.s + 1;
+          st = mg_begin_spec(c);
+          c->solve_key = key;
+          if (st) return st;
+          r.spec = true;
+        }
         int sdone = 0;
-        if ((st = mg_poll(c, block, &sdone))) return st;
+        st = mg_poll(c, block, &sdone);
+        // a continuation of this solve abandoned the stage launched ahead
+        if (r.spec && !c->spec_pending) {
+          r.spec = false;
+          c->spec_abandoned++;
+        }
+        if (st) {
+          r.spec = false;
+          c->spec_pending = false;
+          return st;
+        }
         if (!sdone) return QDEQ_OK;
         if ((st = guess_after_solve(c))) return st;
         if ((st = proj_after_solve(c))) return st;
@@ -821,9 +875,12 @@ static int rk_advance(qdeq_ctx *c, bool block, int *done) {
           }
         } else if (c->rk_ready && !r.final_stage && r.s < 6) {
           // the fault update and stage s + 1's state: one launch, or the
-          // first node of stage s + 1's graph
-          if (!stage_graphs(c)) launch_update_stage(c, r.s);
-          r.staged = true;
+          // first node of stage s + 1's graph, or already queued with
+          // stage s + 1 launched ahead
+          if (!r.spec) {
+            if (!stage_graphs(c)) launch_update_stage(c, r.s);
+            r.staged = true;
+          }
         } else {
           launch_fault_update(c, r.s);
         }
@@ -932,6 +989,13 @@ int qdeq_rk43_step(qdeq_ctx *c, int nsteps, qdeq_step_info *info) {
   c->rk.phase = RK_IDLE;
   if (info) return qdeq_rk43_get_state(c, info, nullptr, nullptr, nullptr,
                                        nullptr, nullptr, nullptr);
+  return QDEQ_OK;
+}
+
+int qdeq_launch_ahead_stats(qdeq_ctx *c, long long *taken, long long *abandoned) {
+  if (!c) return set_error(c, QDEQ_ERR_ARG, "null argument");
+  if (taken) *taken = c->spec_taken;
+  if (abandoned) *abandoned = c->spec_abandoned;
   return QDEQ_OK;
 }
 

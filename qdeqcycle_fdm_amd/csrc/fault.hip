@@ -264,7 +264,10 @@ struct StageArgs {
 };
 
 __global__ void k_update_stage(double *u, int nk, int K, int P, UpdateArgs ua,
-                               StageArgs sa) {
+                               StageArgs sa, const int *spec) {
+  // launched ahead, behind a solve still in flight: runs only if that solve
+  // ended converged (its outcome in spec[2])
+  if (spec && spec[2] != 1) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
   if (k > nk) return;
   update_node(k, u, P, ua.mu_f, ua.tau_n0, ua.Yp, ua.a, ua.b, ua.eta, ua.V0, ua.f0,
@@ -463,7 +466,7 @@ void launch_fault_update(qdeq_ctx *c, int s) {
                      c->c_stress[3], f.Ts, f.Tn, f.FV[s], f.Fd[s], c->flags);
 }
 
-void launch_update_stage(qdeq_ctx *c, int s) {
+void launch_update_stage(qdeq_ctx *c, int s, const int *spec) {
   FaultBufs &f = c->fb;
   const UpdateArgs ua{f.mu_f, f.tau_n0, f.Yp[s], f.a, f.b, f.eta, c->cfg.V0,
                       c->cfg.f0, c->cfg.L, c->c_stress[0], c->c_stress[1],
@@ -472,7 +475,7 @@ void launch_update_stage(qdeq_ctx *c, int s) {
   const StageArgs sa{s + 1, c->scal + SC_T, kC[s + 1], c->cfg.Vini, f.slip, f.psi,
                      f.slip_shift, dev_FV(c), dev_Fd(c), f.Ys[s + 1], f.Yp[s + 1]};
   hipLaunchKernelGGL(k_update_stage, nodes_grid(c->cfg.nk), dim3(FT), 0, c->stream,
-                     c->d[0].u, c->cfg.nk, c->lv[0].K, c->lv[0].P, ua, sa);
+                     c->d[0].u, c->cfg.nk, c->lv[0].K, c->lv[0].P, ua, sa, spec);
 }
 
 void launch_zero_stage_rates(qdeq_ctx *c, double t, double dt) {

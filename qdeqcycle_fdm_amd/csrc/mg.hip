@@ -305,7 +305,9 @@ __global__ __launch_bounds__(NORM_THREADS) void k_start_partials(
     const double *__restrict__ ui, const double *__restrict__ u,
     const double *__restrict__ cand, const double *__restrict__ cx,
     const double *__restrict__ cz, int N, int K, int P, double DH2, int bd,
-    double *__restrict__ partial) {
+    double *__restrict__ partial, const int *__restrict__ spec) {
+  // a start launched ahead: only if the solve before it converged
+  if (spec && spec[2] != 1) return;
   __shared__ double sh[3][NORM_THREADS];
   double acc = 0.0, accc = 0.0, accb = 0.0;
   // each thread's points of a row are evaluated together (their loads in
@@ -364,10 +366,24 @@ __global__ __launch_bounds__(NORM_THREADS) void k_start_partials(
 // stop, gate[1] = 0 V-cycles, gate[2] outcome (1 when the start already
 // meets the threshold, 3 when no V-cycle is allowed), gate[4] candidate
 // used, gate[5] candidate given but not used.  Counts as one step of the
-// mailbox sequence.
+// mailbox sequence.  With spec (a start launched ahead, behind a solve still
+// in flight) and that solve not converged (gate[2] != 1), the start is
+// abandoned: gate[0] = 1 makes the batch behind it skip, and the mailbox
+// step carries outcome 4.
 __global__ __launch_bounds__(NORM_THREADS) void k_start_gate(
     const double *__restrict__ partial, int nb, int cand, int rel, double tol, int cap,
-    double *__restrict__ scal, double *__restrict__ mb, int *__restrict__ gate) {
+    double *__restrict__ scal, double *__restrict__ mb, int *__restrict__ gate, int spec) {
+  if (spec && gate[2] != 1) {
+    if (threadIdx.x == 0) {
+      gate[0] = 1;
+      const int seq = gate[3] + 1;
+      gate[3] = seq;
+      mb[MB_CODE] = 4.0;
+      __threadfence_system();
+      mb[MB_SEQ] = (double)seq;
+    }
+    return;
+  }
   // the three sums of k_norm_final's order, in one tree
   __shared__ double sh[3][NORM_THREADS];
   const int tid = threadIdx.x;
@@ -414,6 +430,10 @@ __global__ __launch_bounds__(NORM_THREADS) void k_start_gate(
     mb[MB_SEQ] = (double)seq;
   }
 }
+
+// A solve that needs more V-cycles after a start was launched ahead of its
+// end: that start was abandoned and stopped the gate; its V-cycles resume.
+__global__ void k_gate_resume(int *gate) { gate[0] = 0; }
 
 // dst = src when flag[0] (the candidate start was chosen).
 __global__ void k_copy_if(double *__restrict__ dst, const double *__restrict__ src,
@@ -563,12 +583,12 @@ static void vcycle(qdeq_ctx *c, LevelBufs<T> *B, int level, bool level0_has_f) {
 static void launch_norm_final(qdeq_ctx *c, int nparts) {
   if (c->gate_ptr)
     hipLaunchKernelGGL(k_norm_gate, dim3(1), dim3(NORM_THREADS), 0, c->stream,
-                       c->partial, nparts, c->scal, c->mbox_dev, c->gate_ptr,
+                       c->partial, nparts, c->scal, mb_dev(c), c->gate_ptr,
                        c->cfg.tol_mode == QDEQ_TOL_REL ? 1 : 0, c->cfg.tol,
                        c->cfg.max_vcycles);
   else
     hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(NORM_THREADS), 0, c->stream,
-                       c->partial, nparts, c->scal + SC_NORM, c->mbox_dev + SC_NORM);
+                       c->partial, nparts, c->scal + SC_NORM, mb_dev(c) + SC_NORM);
 }
 
 // Level-0 residual norm into scal[SC_NORM] (and r32/e32 for IR).
@@ -598,7 +618,7 @@ void launch_resnorm_of(qdeq_ctx *c, const double *u, int off, int slot) {
                      (float *)nullptr, (float *)nullptr, (const int *)nullptr);
   hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(NORM_THREADS), 0, c->stream,
                      c->partial + off, norm_blocks(c), c->scal + slot,
-                     c->mbox_dev + slot);
+                     mb_dev(c) + slot);
 }
 
 static void launch_bdres(qdeq_ctx *c) {
@@ -609,7 +629,7 @@ static void launch_bdres(qdeq_ctx *c) {
                      d.DH * d.DH, c->partial + NORM_BLOCKS);
   hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(NORM_THREADS), 0, c->stream,
                      c->partial + NORM_BLOCKS, nb, c->scal + SC_NORM0,
-                     c->mbox_dev + SC_NORM0);
+                     mb_dev(c) + SC_NORM0);
 }
 
 // Whether solves of this context decide their start on the device: tiled
@@ -694,11 +714,11 @@ constexpr double kMboxEmpty = -1.0;
 constexpr long long kTimingEvery = 8;
 
 void mbox_arm(qdeq_ctx *c, int slot) {
-  reinterpret_cast<volatile double *>(c->mbox_host)[slot] = kMboxEmpty;
+  reinterpret_cast<volatile double *>(mb_host(c))[slot] = kMboxEmpty;
 }
 
 int mbox_wait(qdeq_ctx *c, int slot) {
-  volatile double *p = reinterpret_cast<volatile double *>(c->mbox_host) + slot;
+  volatile double *p = reinterpret_cast<volatile double *>(mb_host(c)) + slot;
   long spins = 0;
   std::chrono::steady_clock::time_point idle_since{};
   bool idle = false;
@@ -735,7 +755,7 @@ static int mbox_check(qdeq_ctx *c, int slot, bool block, int *ready) {
     *ready = st == QDEQ_OK;
     return st;
   }
-  volatile double *p = reinterpret_cast<volatile double *>(c->mbox_host) + slot;
+  volatile double *p = reinterpret_cast<volatile double *>(mb_host(c)) + slot;
   if (*p != kMboxEmpty) {
     std::atomic_thread_fence(std::memory_order_acquire);
     c->mb_idle = false;
@@ -768,6 +788,7 @@ static int mbox_check(qdeq_ctx *c, int slot, bool block, int *ready) {
 // blocking unless asked to.  A solve runs the same launches in the same order
 // whether it is polled or waited for.
 static int launch_batch(qdeq_ctx *c, int k);
+static int launch_batch_of(qdeq_ctx *c, MgSolve &m, int k, bool allow_sample);
 
 // The device start of a solve without a candidate: FP64 folds it into the
 // first V-cycle's down kernels when that fits (fuse), else runs the start
@@ -793,11 +814,11 @@ static void launch_plain_dev_start(qdeq_ctx *c, bool fuse) {
     hipLaunchKernelGGL(k_start_partials<false>, dim3(norm_blocks(c)), dim3(NORM_THREADS),
                        0, c->stream, c->d[0].u, c->d[0].u, (const double *)nullptr,
                        c->d[0].cx, c->d[0].cz, d.N, d.K, d.P, d.DH * d.DH, rel,
-                       c->partial);
+                       c->partial, (const int *)(c->spec_launch ? c->gate : nullptr));
   }
   hipLaunchKernelGGL(k_start_gate, dim3(1), dim3(NORM_THREADS), 0, c->stream,
                      c->partial, norm_blocks(c), 0, rel, c->cfg.tol, c->cfg.max_vcycles,
-                     c->scal, c->mbox_dev, c->gate);
+                     c->scal, mb_dev(c), c->gate, c->spec_launch ? 1 : 0);
 }
 
 // V-cycles of the first batch of a solve: what the same RK43 stage needed in
@@ -813,6 +834,9 @@ int mg_begin(qdeq_ctx *c) {
   int st = ensure_graph(c);
   if (st != QDEQ_OK) return st;
   MgSolve &m = c->mg;
+  // no other solve is in flight: mailbox page 0
+  m.page = 0;
+  c->mb_page = 0;
   m.ir = c->cfg.precision == QDEQ_PREC_FP32_IR;
   // a predicted start (learned guess) was enqueued before this solve: its
   // candidate sits in the level-0 r buffer, its residual norm in
@@ -854,10 +878,10 @@ int mg_begin(qdeq_ctx *c) {
     } else {
       hipLaunchKernelGGL(k_start_partials<true>, dim3(norm_blocks(c)), dim3(NORM_THREADS), 0,
                          c->stream, plain, c->d[0].u, cand, c->d[0].cx, c->d[0].cz,
-                         d.N, d.K, d.P, d.DH * d.DH, rel, c->partial);
+                         d.N, d.K, d.P, d.DH * d.DH, rel, c->partial, (const int *)nullptr);
       hipLaunchKernelGGL(k_start_gate, dim3(1), dim3(NORM_THREADS), 0, c->stream,
                          c->partial, norm_blocks(c), 1, rel, c->cfg.tol,
-                         c->cfg.max_vcycles, c->scal, c->mbox_dev, c->gate);
+                         c->cfg.max_vcycles, c->scal, mb_dev(c), c->gate, 0);
     }
     if (in_u) {
       // the plain start back over a candidate that was not chosen
@@ -904,6 +928,8 @@ int mg_begin_stage(qdeq_ctx *c, int kind, int s) {
   int st = ensure_graph(c);
   if (st != QDEQ_OK) return st;
   MgSolve &m = c->mg;
+  m.page = 0;
+  c->mb_page = 0;
   m.ir = c->cfg.precision == QDEQ_PREC_FP32_IR;
   m.cand = false;
   c->cand_pending = false;
@@ -947,6 +973,34 @@ int mg_begin_stage(qdeq_ctx *c, int kind, int s) {
   return QDEQ_OK;
 }
 
+// The next FP64 solve launched ahead, behind the current solve's launched
+// V-cycles (its fault update already queued by the caller): the plain
+// device start and a first batch, into c->mg_next, on the other mailbox
+// page.  Its kernels go on only if the current solve ends converged with
+// the work already queued; if the current solve needs more V-cycles the
+// start abandons itself (outcome 4) and mg_poll resumes the gate before the
+// continuation.  The same kernels and arguments as mg_begin's, so a solve
+// launched ahead and taken over computes what it would have computed.
+int mg_begin_spec(qdeq_ctx *c) {
+  MgSolve &m = c->mg_next;
+  m = MgSolve{};
+  m.dev_start = true;
+  m.page = 1 - c->mg.page;
+  c->mb_page = m.page;
+  c->spec_launch = true;
+  launch_plain_dev_start(c, fused_start_fits(c));
+  c->seq_issued += 1;
+  const int st = launch_batch_of(c, m, first_batch(c), false);
+  c->spec_launch = false;
+  c->fuse_start_next = false;
+  c->mb_page = c->mg.page;
+  if (st != QDEQ_OK) return st;
+  QDEQ_HIP(c, hipGetLastError());
+  m.phase = 2;
+  c->spec_pending = true;
+  return QDEQ_OK;
+}
+
 static int mg_finish(qdeq_ctx *c, int st) {
   MgSolve &m = c->mg;
   c->vcycles += m.it;
@@ -967,7 +1021,7 @@ static int mg_finish(qdeq_ctx *c, int st) {
 // Non-blocking (or blocking) check that the batch's last V-cycle has been
 // seen by the device: mb[MB_SEQ] reaches the batch's sequence number.
 static int seq_check(qdeq_ctx *c, long long target, bool block, int *ready) {
-  volatile double *p = reinterpret_cast<volatile double *>(c->mbox_host) + MB_SEQ;
+  volatile double *p = reinterpret_cast<volatile double *>(mb_host(c)) + MB_SEQ;
   for (;;) {
     if ((long long)*p >= target) {
       std::atomic_thread_fence(std::memory_order_acquire);
@@ -998,14 +1052,18 @@ static int seq_check(qdeq_ctx *c, long long target, bool block, int *ready) {
 
 // Launches up to k V-cycles back to back; each skips once the solve has
 // converged (or failed), so launching more than needed changes nothing.
-static int launch_batch(qdeq_ctx *c, int k) {
-  MgSolve &m = c->mg;
+static int launch_batch(qdeq_ctx *c, int k) { return launch_batch_of(c, c->mg, k, true); }
+
+// The batch of solve m (the current one, or one launched ahead, whose
+// V-cycles are never timing samples: two solves in flight would share the
+// events).
+static int launch_batch_of(qdeq_ctx *c, MgSolve &m, int k, bool allow_sample) {
   m.sample = false;
   for (int i = 0; i < k; i++) {
     // kernel timing samples every kTimingEvery-th V-cycle: that one is
     // launched directly (same kernels and arguments as the graph) with the
     // events recorded around the timed kernel
-    const bool sample = c->timing != 0 && c->timing != QDEQ_TIME_FAULT &&
+    const bool sample = allow_sample && c->timing != 0 && c->timing != QDEQ_TIME_FAULT &&
                         use_tiled(c) && (c->timed_seq++ % kTimingEvery) == 0;
     m.launched++;
     if (sample) {
@@ -1045,6 +1103,8 @@ static int batch_size(const qdeq_ctx *c, double delta, double thr) {
 
 int mg_poll(qdeq_ctx *c, bool block, int *done) {
   MgSolve &m = c->mg;
+  c->mb_page = m.page;
+  const double *mb = mb_host(c);
   *done = 0;
   int st = QDEQ_OK, ready;
   if (m.phase == 1) {
@@ -1054,7 +1114,7 @@ int mg_poll(qdeq_ctx *c, bool block, int *done) {
       goto out;
     if (m.cand && ((st = mbox_check(c, SC_NORM_PRED, block, &ready)) || !ready))
       goto out;
-    for (int i : {SC_NORM, SC_NORM0, SC_NORM_PRED}) c->scal_host[i] = c->mbox_host[i];
+    for (int i : {SC_NORM, SC_NORM0, SC_NORM_PRED}) c->scal_host[i] = mb[i];
     m.delta = c->scal_host[SC_NORM];
     if (m.cand) {
       // used only if its residual is finite and below the previous solution's
@@ -1094,7 +1154,7 @@ int mg_poll(qdeq_ctx *c, bool block, int *done) {
       // before it)
       float ms = 0.0f;
       QDEQ_HIP(c, hipEventSynchronize(c->ev_t1));
-      if ((int)c->mbox_host[MB_COUNT] >= m.sample_ordinal &&
+      if ((int)mb[MB_COUNT] >= m.sample_ordinal &&
           hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1) == hipSuccess) {
         c->timed_ms += ms;
         c->timed_calls += 1;
@@ -1103,20 +1163,20 @@ int mg_poll(qdeq_ctx *c, bool block, int *done) {
     }
     if (m.dev_start && !m.start_read) {
       // the start's norms, written before the first sequence step
-      const double r0 = c->mbox_host[SC_NORM0];
-      m.delta0 = c->mbox_host[MB_START];
+      const double r0 = mb[SC_NORM0];
+      m.delta0 = mb[MB_START];
       m.r0 = c->cfg.tol_mode == QDEQ_TOL_REL ? r0 : m.delta0;
       m.thr = c->cfg.tol_mode == QDEQ_TOL_REL ? c->cfg.tol * r0 : c->cfg.tol;
       c->scal_host[SC_NORM0] = r0;
       if (m.cand) {
-        c->scal_host[SC_NORM_PRED] = c->mbox_host[SC_NORM_PRED];
-        if (c->mbox_host[MB_USED] != 0.0) c->g_accepted++;
+        c->scal_host[SC_NORM_PRED] = mb[SC_NORM_PRED];
+        if (mb[MB_USED] != 0.0) c->g_accepted++;
       }
       m.start_read = true;
     }
-    const int code = (int)c->mbox_host[MB_CODE];
-    m.it = (int)c->mbox_host[MB_COUNT];
-    m.delta = c->scal_host[SC_NORM] = c->mbox_host[SC_NORM];
+    const int code = (int)mb[MB_CODE];
+    m.it = (int)mb[MB_COUNT];
+    m.delta = c->scal_host[SC_NORM] = mb[SC_NORM];
     if (code == 2) {
       *done = 1;
       return mg_finish(c, set_error(c, QDEQ_ERR_MG_NOT_CONVERGED,
@@ -1133,7 +1193,12 @@ int mg_poll(qdeq_ctx *c, bool block, int *done) {
       *done = 1;
       return mg_finish(c, QDEQ_OK);
     }
-    // not there yet: another batch sized from the residual left
+    // not there yet: another batch sized from the residual left.  A solve
+    // launched ahead behind this one was abandoned and stopped the gate.
+    if (c->spec_pending) {
+      hipLaunchKernelGGL(k_gate_resume, dim3(1), dim3(1), 0, c->stream, c->gate);
+      c->spec_pending = false;
+    }
     if ((st = launch_batch(c, batch_size(c, m.delta, m.thr)))) goto out;
     if (!block) return QDEQ_OK;
   }

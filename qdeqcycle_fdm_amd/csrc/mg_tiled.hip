@@ -312,6 +312,11 @@ struct StartArgs {
   int *gate;
   int rel, cap;
   double tol;
+  // launched ahead, behind a solve still in flight: START_PARTIALS goes on
+  // only if that solve converged (gate[2] == 1) and records the verdict in
+  // gate[6]; START_TEST abandons the start when gate[6] is 0 (gate[0] = 1
+  // stops the batch behind it, the mailbox step carries outcome 4)
+  int spec;
 };
 
 // Pre-smoothing + residual + restriction (levels above the tail).
@@ -322,6 +327,24 @@ __global__ __launch_bounds__(TILE_THREADS) void k_down(
     T DH2, T relax, int nu, int zero_u, T *__restrict__ r2, int Nc, int Kc,
     int Pc, int TX, int TY, const int *__restrict__ gate, StartArgs sa) {
   if (SMODE == DOWN_PLAIN && gate && gate[0]) return;  // converged: skip
+  const bool wg0 = blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 &&
+                   threadIdx.y == 0;
+  if (SMODE == START_PARTIALS && sa.spec) {
+    const int ok = sa.gate[2] == 1 ? 1 : 0;
+    if (wg0) sa.gate[6] = ok;
+    if (!ok) return;
+  }
+  if (SMODE == START_TEST && sa.spec && sa.gate[6] == 0) {
+    if (wg0) {
+      sa.gate[0] = 1;
+      const int seq = sa.gate[3] + 1;
+      sa.gate[3] = seq;
+      sa.mb[MB_CODE] = 4.0;
+      __threadfence_system();
+      sa.mb[MB_SEQ] = (double)seq;
+    }
+    return;
+  }
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ double sdec[16 * 2];
   if (SMODE == START_TEST && threadIdx.y == 0) {
@@ -860,8 +883,9 @@ static void launch_down(qdeq_ctx *c, LevelBufs<T> *B, int l, bool has_f,
     sa.part = c->partial + kStartOff;
     sa.nb = ((d0.K + s0.tx - 1) / s0.tx) * ((d0.N + s0.ty - 1) / s0.ty);
     sa.scal = c->scal;
-    sa.mb = c->mbox_dev;
+    sa.mb = mb_dev(c);
     sa.gate = c->gate;
+    sa.spec = c->spec_launch ? 1 : 0;
     sa.rel = c->cfg.tol_mode == QDEQ_TOL_REL ? 1 : 0;
     sa.cap = c->cfg.max_vcycles;
     sa.tol = c->cfg.tol;

@@ -84,6 +84,7 @@ struct MgSolve {
   int it = 0;
   double delta = 0, delta0 = 0, r0 = 0, thr = 0;
   long long seq_target = 0;  // sequence number ending the launched batch
+  int page = 0;              // mailbox page its kernels write (see mb_dev)
 };
 
 // State of the resumable RK43 driver (api.hip).
@@ -94,6 +95,9 @@ struct RkMachine {
   int attempts = 0;        // attempts of the current step
   long long steps_left = 0;
   bool final_stage = false;  // stage 1 after an accepted step
+  // the next stage's fault update, start and first batch are queued behind
+  // stage s's solve (qdeq_ctx::mg_next), to be taken over if it converges
+  bool spec = false;
   hipEvent_t ev = nullptr;   // completion of the last host readback
 };
 
@@ -134,6 +138,17 @@ struct qdeq_ctx {
   int launch_env = 0;
   bool in_ensemble = false;
   bool host_start = false;     // start decisions on the host (see mg_begin)
+  // Solves launched ahead (see spec_allowed in api.hip): the solve queued
+  // behind the current one, whether it is still live (not abandoned by a
+  // continuation of the current solve), the mailbox page of the kernels
+  // being launched (two solves in flight write different pages), and
+  // whether the launches in progress are such a solve's.
+  qdeq::MgSolve mg_next;
+  bool spec_pending = false;
+  int mb_page = 0;
+  bool spec_launch = false;
+  bool spec_off = false;       // QDEQ_SPECULATE=0
+  long long spec_taken = 0, spec_abandoned = 0;
   bool stage_graphs_off = false;  // QDEQ_STAGE_GRAPHS=0 (see stage_graphs)
   int last_it = 2;             // V-cycles of the last solve (first batch size)
   // RK43 solves: the stage's key (stage s, +8 for the solve after an
@@ -237,6 +252,7 @@ int mg_begin(qdeq_ctx *c);
 // kind 1: stage s - 1's fault update and stage s's state)
 bool stage_graphs(const qdeq_ctx *c);
 int mg_begin_stage(qdeq_ctx *c, int kind, int s);
+int mg_begin_spec(qdeq_ctx *c);
 int mg_poll(qdeq_ctx *c, bool block, int *done);
 int mg_build_coefficients(qdeq_ctx *c);
 void mg_destroy_graphs(qdeq_ctx *c);
@@ -378,6 +394,11 @@ inline int norm_blocks(const qdeq_ctx *c) {
   return c->lv[0].N < kNormCap ? c->lv[0].N : kNormCap;
 }
 
+// Host-mapped mailbox: two pages of SC_COUNT doubles; the kernels being
+// launched, and the host reads of the solve being polled, use page mb_page.
+inline double *mb_dev(const qdeq_ctx *c) { return c->mbox_dev + c->mb_page * SC_COUNT; }
+inline double *mb_host(const qdeq_ctx *c) { return c->mbox_host + c->mb_page * SC_COUNT; }
+
 inline bool launch_direct(const qdeq_ctx *c) {
   return c->launch_env == 1 || (c->launch_env == 0 && !c->in_ensemble);
 }
@@ -409,7 +430,7 @@ void launch_dpsi(qdeq_ctx *c, const double *psi, const double *V,
 // stage kernels: t and dt from scal[SC_T], scal[SC_DT]
 void launch_stage(qdeq_ctx *c, int s);
 // fault update of stage s and the stage state of s + 1 in one launch
-void launch_update_stage(qdeq_ctx *c, int s);
+void launch_update_stage(qdeq_ctx *c, int s, const int *spec = nullptr);
 void launch_fault_update(qdeq_ctx *c, int s);
 // zeroes the stage rates of a new attempt and sets its t, dt on the device
 void launch_zero_stage_rates(qdeq_ctx *c, double t, double dt);

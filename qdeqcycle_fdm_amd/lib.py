@@ -44,7 +44,7 @@ EXPORTS = (
     "qdeq_kernel_breakdown", "qdeq_set_tile_shapes", "qdeq_ensemble_step",
     "qdeq_output_open", "qdeq_output_close", "qdeq_events_enable",
     "qdeq_events_get", "qdeq_projection_enable", "qdeq_projection_stats",
-    "qdeq_projection_fit", "qdeq_projection_inject",
+    "qdeq_projection_fit", "qdeq_projection_inject", "qdeq_launch_ahead_stats",
     # include/qdeq_guess.h
     "qdeq_guess_default_config", "qdeq_train_default_config",
     "qdeq_guess_create", "qdeq_guess_destroy", "qdeq_guess_last_error",
@@ -231,6 +231,7 @@ def load(path: str = LIB_PATH):
     L.qdeq_guess_infer.argtypes = [_vp, _fp, _fp]
     L.qdeq_projection_enable.argtypes = [_vp, ctypes.c_int]
     L.qdeq_projection_stats.argtypes = [_vp, _llp, _llp, _dp, _ip]
+    L.qdeq_launch_ahead_stats.argtypes = [_vp, _llp, _llp]
     L.qdeq_projection_fit.argtypes = [_vp, _dp, _dp, ctypes.c_int, _dp]
     L.qdeq_projection_inject.argtypes = [_vp, ctypes.c_double]
     for name in EXPORTS:
@@ -740,6 +741,17 @@ Engine.guess_stats = _engine_guess_stats
 Engine.guess_inject = _engine_guess_inject
 Engine.projection_enable = _engine_projection_enable
 Engine.projection_stats = _engine_projection_stats
+
+
+def _engine_launch_ahead_stats(self) -> dict:
+    """Solves launched ahead behind the previous stage's solve and taken
+    over, or abandoned (the previous solve needed more V-cycles)."""
+    t, a = ctypes.c_longlong(0), ctypes.c_longlong(0)
+    self._check(load().qdeq_launch_ahead_stats(self.h, ctypes.byref(t), ctypes.byref(a)))
+    return dict(taken=t.value, abandoned=a.value)
+
+
+Engine.launch_ahead_stats = _engine_launch_ahead_stats
 Engine.projection_fit = _engine_projection_fit
 Engine.projection_inject = lambda self, factor: self._check(
     load().qdeq_projection_inject(self.h, factor))

@@ -388,3 +388,39 @@ def test_ir_device_start_equals_host_start(n, levels, monkeypatch):
         assert a["info"][key] == b["info"][key], key
     for k in ("slip", "psi", "V", "U"):
         np.testing.assert_array_equal(a[k], b[k])
+
+
+@pytest.mark.parametrize("n,levels", [(257, 5), (65, 3)])
+@pytest.mark.parametrize("timed", [False, True])
+def test_stages_launched_ahead_equal_waiting_runs(n, levels, timed, monkeypatch):
+    """One simulation queues each stage's fault update, start and first
+    V-cycles behind the previous stage's solve; they run only if that solve
+    converged with the V-cycles already queued, else they abandon themselves
+    and the solve continues.  The runs are bit for bit those of
+    QDEQ_SPECULATE=0 (every stage waits), V-cycle counts included, with the
+    folded start (257^2) and the separate start kernels (65^2), with and
+    without kernel timing; stages are taken over and some abandoned."""
+    pb = config.make_bp1_problem(n, basin=True, levels=levels,
+                                 tol_mode=config.TOL_REL, tol=1e-10)
+    me = oracle.mu_equi(pb.mu)
+    runs = []
+    for spec in (True, False):
+        if spec:
+            monkeypatch.delenv("QDEQ_SPECULATE", raising=False)
+        else:
+            monkeypatch.setenv("QDEQ_SPECULATE", "0")
+        with lib.Engine(pb, me, **PROD) as eng:
+            eng.set_tiles(True, 64, 16)
+            eng.rk43_setup()
+            if timed:
+                eng.set_kernel_timing(lib.TIME_UP0)
+            eng.rk43_step(120)
+            runs.append((eng.state(), eng.launch_ahead_stats()))
+    (a, sa), (b, sb) = runs
+    for key in ("t", "dt", "vcycles", "solves", "attempts"):
+        assert a["info"][key] == b["info"][key], key
+    for k in ("slip", "psi", "V", "U"):
+        np.testing.assert_array_equal(a[k], b[k])
+    assert sb == {"taken": 0, "abandoned": 0}
+    assert sa["taken"] > 0.5 * a["info"]["solves"]
+    assert sa["abandoned"] > 0
