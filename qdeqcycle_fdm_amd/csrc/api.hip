@@ -102,8 +102,9 @@ static void free_ctx(qdeq_ctx *c) {
   if (c->scal_host) (void)hipHostFree(c->scal_host);
   if (c->mbox_host) (void)hipHostFree(c->mbox_host);
   if (c->flags_host) (void)hipHostFree(c->flags_host);
-  if (c->ev_t0) (void)hipEventDestroy(c->ev_t0);
-  if (c->ev_t1) (void)hipEventDestroy(c->ev_t1);
+  for (auto &pair : c->ev_t)
+    for (hipEvent_t e : pair)
+      if (e) (void)hipEventDestroy(e);
   guess_free_ctx(c);
   proj_free(c);
   (void)qdeq_output_close(c);
@@ -337,9 +338,9 @@ int qdeq_device_synchronize(int device) {
 int qdeq_set_kernel_timing(qdeq_ctx *c, int which) {
   if (!c || which < 0 || which > QDEQ_TIME_FAULT)
     return set_error(c, QDEQ_ERR_ARG, "bad timing selector");
-  if (which && !c->ev_t0) {
-    QDEQ_HIP(c, hipEventCreate(&c->ev_t0));
-    QDEQ_HIP(c, hipEventCreate(&c->ev_t1));
+  if (which && !c->ev_t[0][0]) {
+    for (auto &pair : c->ev_t)
+      for (hipEvent_t &e : pair) QDEQ_HIP(c, hipEventCreate(&e));
   }
   c->timing = which;
   c->timed_ms = 0.0;
@@ -864,12 +865,12 @@ static int rk_advance(qdeq_ctx *c, bool block, int *done) {
         if ((st = guess_after_solve(c))) return st;
         if ((st = proj_after_solve(c))) return st;
         if (c->timing == QDEQ_TIME_FAULT && (c->timed_seq++ % 8) == 0) {
-          QDEQ_HIP(c, hipEventRecord(c->ev_t0, c->stream));
+          QDEQ_HIP(c, hipEventRecord(c->ev_t[0][0], c->stream));
           launch_fault_update(c, r.s);
-          QDEQ_HIP(c, hipEventRecord(c->ev_t1, c->stream));
-          QDEQ_HIP(c, hipEventSynchronize(c->ev_t1));
+          QDEQ_HIP(c, hipEventRecord(c->ev_t[0][1], c->stream));
+          QDEQ_HIP(c, hipEventSynchronize(c->ev_t[0][1]));
           float ms = 0.0f;
-          if (hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1) == hipSuccess) {
+          if (hipEventElapsedTime(&ms, c->ev_t[0][0], c->ev_t[0][1]) == hipSuccess) {
             c->timed_ms += ms;
             c->timed_calls += 1;
           }

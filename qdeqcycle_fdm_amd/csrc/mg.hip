@@ -990,7 +990,7 @@ int mg_begin_spec(qdeq_ctx *c) {
   c->spec_launch = true;
   launch_plain_dev_start(c, fused_start_fits(c));
   c->seq_issued += 1;
-  const int st = launch_batch_of(c, m, first_batch(c), false);
+  const int st = launch_batch_of(c, m, first_batch(c), true);
   c->spec_launch = false;
   c->fuse_start_next = false;
   c->mb_page = c->mg.page;
@@ -1054,9 +1054,9 @@ static int seq_check(qdeq_ctx *c, long long target, bool block, int *ready) {
 // converged (or failed), so launching more than needed changes nothing.
 static int launch_batch(qdeq_ctx *c, int k) { return launch_batch_of(c, c->mg, k, true); }
 
-// The batch of solve m (the current one, or one launched ahead, whose
-// V-cycles are never timing samples: two solves in flight would share the
-// events).
+// The batch of solve m (the current one, or one launched ahead; each
+// mailbox page has its own pair of timing events, so either may hold a
+// sample).
 static int launch_batch_of(qdeq_ctx *c, MgSolve &m, int k, bool allow_sample) {
   m.sample = false;
   for (int i = 0; i < k; i++) {
@@ -1153,9 +1153,9 @@ int mg_poll(qdeq_ctx *c, bool block, int *done) {
       // counted only if the timed V-cycle ran (the solve had not stopped
       // before it)
       float ms = 0.0f;
-      QDEQ_HIP(c, hipEventSynchronize(c->ev_t1));
+      QDEQ_HIP(c, hipEventSynchronize(c->ev_t[m.page][1]));
       if ((int)mb[MB_COUNT] >= m.sample_ordinal &&
-          hipEventElapsedTime(&ms, c->ev_t0, c->ev_t1) == hipSuccess) {
+          hipEventElapsedTime(&ms, c->ev_t[m.page][0], c->ev_t[m.page][1]) == hipSuccess) {
         c->timed_ms += ms;
         c->timed_calls += 1;
       }

@@ -192,7 +192,8 @@ struct qdeq_ctx {
   // directly instead of as a graph (emit_events) with events around it
   int timing = 0;
   bool emit_events = false;
-  hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;
+  // one pair per mailbox page: a solve launched ahead may hold a sample too
+  hipEvent_t ev_t[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   hipEvent_t ev_caller = nullptr;  // the caller's null-stream work (QDEQ_DEVICE)
   double timed_ms = 0.0;
   long long timed_calls = 0, timed_seq = 0;
@@ -301,7 +302,7 @@ int kernel_breakdown(qdeq_ctx *c, int reps, int *n, double *us);
 // sequence: records the timing events on a directly launched V-cycle
 inline void timing_mark(qdeq_ctx *c, int which, int pos) {
   if (c->emit_events && c->timing == which)
-    (void)hipEventRecord(pos ? c->ev_t1 : c->ev_t0, c->stream);
+    (void)hipEventRecord(c->ev_t[c->mb_page][pos], c->stream);
 }
 
 // exact coarse solve (coarse.hip).  The matrix-vector product splits each
