@@ -261,6 +261,20 @@ __device__ inline T tile_residual(const Tile<T> &t, int lr, int lc, int j,
                   DH2;
 }
 
+// Tile handled by this workgroup.  Workgroups are dispatched to the 8 XCDs
+// in turn (linear id mod 8), and each XCD has its own L2; the tiles are
+// dealt out in 8 contiguous bands of row-major tile order, one per XCD, so
+// neighbouring tiles (their halos overlap) and the tiles of consecutive
+// levels over the same part of the grid run on the same XCD.  XCD x holds
+// q + (x < r) workgroups for n = 8 q + r tiles.
+__device__ inline int tile_id() {
+  const int n = gridDim.x * gridDim.y;
+  const int id = blockIdx.y * gridDim.x + blockIdx.x;
+  const int x = id & 7, slot = id >> 3;
+  const int q = n >> 3, r = n & 7;
+  return x * q + min(x, r) + slot;
+}
+
 template <typename T>
 __device__ inline Tile<T> make_tile(unsigned char *smem, int TX, int TY, int H,
                                     bool has_f) {
@@ -268,8 +282,9 @@ __device__ inline Tile<T> make_tile(unsigned char *smem, int TX, int TY, int H,
   t.R = TY + 2 * H;
   t.C = TX + 2 * H;
   t.LS = t.C;
-  t.j0 = (int)blockIdx.y * TY - H;
-  t.k0 = (int)blockIdx.x * TX - H;
+  const int tile = tile_id();
+  t.j0 = (tile / (int)gridDim.x) * TY - H;
+  t.k0 = (tile % (int)gridDim.x) * TX - H;
   T *p = reinterpret_cast<T *>(smem);
   const int n = t.R * t.LS;
   t.su = p;
@@ -411,7 +426,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_down(
     }
     block_sums<2>(v2, sdec);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
-      const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+      const int bid = tile_id();
       sa.part[bid] = v2[0];
       sa.part[kStartCap + bid] = v2[1];
     }
@@ -531,7 +546,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_up(
     __shared__ double red[16];
     const double s = block_sum1(acc, red);
     if (threadIdx.x == 0 && threadIdx.y == 0)
-      partial[blockIdx.y * gridDim.x + blockIdx.x] = s;
+      partial[tile_id()] = s;
   }
 }
 
