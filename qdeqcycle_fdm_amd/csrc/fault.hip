@@ -333,7 +333,10 @@ __global__ __launch_bounds__(1024) void k_combine(
     }
     __syncthreads();
     if (tid == 0) {
+      // the sums in node order; unrolled so the LDS loads issue ahead of
+      // the dependent additions
       const int n = (nk - base + 1) < 1024 ? (nk - base + 1) : 1024;
+#pragma unroll 16
       for (int i = 0; i < n; i++) {
         e1 = e1 + s1[i];
         e2 = e2 + s2[i];
