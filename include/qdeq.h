@@ -304,10 +304,15 @@ int qdeq_projection_stats(qdeq_ctx *ctx, long long *offered,
  * one simulation launching kernel by kernel, each stage's fault update,
  * start and first V-cycles are queued behind the previous stage's solve and
  * run only if that solve converged with the V-cycles already queued; else
- * they abandon themselves and the solve goes on.  Results are the same
- * either way (QDEQ_SPECULATE=0 in the environment at qdeq_create turns it
- * off).  Counts of such solves taken over and abandoned. */
-int qdeq_launch_ahead_stats(qdeq_ctx *ctx, long long *taken, long long *abandoned);
+ * they abandon themselves and the solve goes on.  The same holds across
+ * steps (the next step's stage 2 behind an accepted step's final stage 1)
+ * when the next step runs within the same call and neither per-step outputs
+ * nor events are on.  Results are the same either way (QDEQ_SPECULATE=0 in
+ * the environment at qdeq_create turns it off).  Counts of such solves taken
+ * over and abandoned, and of those taken, the ones across a step (any
+ * pointer may be NULL). */
+int qdeq_launch_ahead_stats(qdeq_ctx *ctx, long long *taken, long long *abandoned,
+                            long long *taken_next_step);
 
 /* Copies the current state to host arrays (any may be NULL): U (nx*nz, with
  * ghosts as ode_solve leaves them), slip, psi, V, dpsi, Ts (nz). */

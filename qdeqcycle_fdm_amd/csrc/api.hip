@@ -693,7 +693,7 @@ int qdeq_rk43_setup(qdeq_ctx *c, const double *a, const double *b,
   c->errA[0] = c->errA[1] = 0.0;
   c->totErr = 0.0;
   c->vcycles = c->solves = c->ir_sweeps = 0;
-  c->spec_taken = c->spec_abandoned = 0;
+  c->spec_taken = c->spec_abandoned = c->spec_taken_next = 0;
   c->rk_ready = false;
   c->rk.phase = 0;  // idle
   c->rk.staged = false;
@@ -772,6 +772,24 @@ static bool spec_allowed(const qdeq_ctx *c) {
          c->mg.phase == 2 && c->mg.dev_start && !c->mg.cand && !c->spec_pending;
 }
 
+// The step size an accepted step hands to the next one (odeSolver.cpp:1621-1632)
+static double step_finish_dt(const qdeq_ctx *c) {
+  const qdeq_config &p = c->cfg;
+  if (c->totErr != 0.0)
+    return qdeq_compute_step_size(c->it, c->dt, p.mindt, p.maxdt, c->totErr, p.totTol,
+                                  c->errA);
+  return c->dt;
+}
+
+// Whether stage 2 of the next step may be queued behind an accepted step's
+// final stage-1 solve: the next step runs within this request and nothing is
+// recorded between the steps (per-step outputs and events read the state
+// there).
+static bool spec_next_allowed(const qdeq_ctx *c) {
+  return c->rk.steps_left > 1 && c->t < c->cfg.tmax && !c->out && !c->ev_on &&
+         spec_allowed(c);
+}
+
 // Advances the machine as far as it can; *done = 1 when the requested steps
 // (and the final flag check) are complete.  With block, every wait blocks.
 static int rk_advance(qdeq_ctx *c, bool block, int *done) {
@@ -789,6 +807,8 @@ static int rk_advance(qdeq_ctx *c, bool block, int *done) {
       case RK_STEP_BEGIN:
         // while (it < maxNumSteps && t_elapsed < tmax) (odeSolver.cpp:1462)
         if (r.steps_left <= 0 || !(c->t < p.tmax)) {
+          if (r.spec)
+            return set_error(c, QDEQ_ERR_STATE, "a stage is queued for a step that does not run");
           if ((st = rk_readback(c, false))) return st;
           r.phase = RK_WAIT_END;
           break;
@@ -800,7 +820,14 @@ static int rk_advance(qdeq_ctx *c, bool block, int *done) {
       case RK_ATTEMPT_BEGIN:  // odeSolver.cpp:1466-1604
         r.attempts = r.attempts + 1;
         if (c->t + c->dt > p.tmax) c->dt = p.tmax - c->t;
-        launch_zero_stage_rates(c, c->t, c->dt);
+        if (r.spec) {
+          // stage 2 already queued behind the last step's final solve, with
+          // the dt computed there
+          if (c->dt != r.spec_dt)
+            return set_error(c, QDEQ_ERR_STATE, "step size differs from the queued stage's");
+        } else {
+          launch_zero_stage_rates(c, c->t, c->dt);
+        }
         r.s = 2;
         r.final_stage = false;
         r.phase = RK_STAGE_BEGIN;
@@ -814,6 +841,7 @@ static int rk_advance(qdeq_ctx *c, bool block, int *done) {
           c->mb_page = c->mg.page;
           c->spec_pending = false;
           c->spec_taken++;
+          if (r.s == 2 && !r.final_stage) c->spec_taken_next++;
           r.spec = false;
           r.staged = false;
           r.phase = RK_SOLVING;
@@ -848,6 +876,20 @@ static int rk_advance(qdeq_ctx *c, bool block, int *done) {
           c->solve_key = key;
           if (st) return st;
           r.spec = true;
+        } else if (!r.spec && c->rk_ready && r.final_stage && spec_next_allowed(c)) {
+          // the next step's first attempt queued behind this solve: this
+          // stage's fault update, the attempt's zeroed rates and stage 2's
+          // state, start and first batch, with the dt the step will hand on
+          double dt = step_finish_dt(c);
+          if (c->t + dt > p.tmax) dt = p.tmax - c->t;
+          launch_update_next(c, c->t, dt, c->gate);
+          const int key = c->solve_key;
+          c->solve_key = 2;
+          st = mg_begin_spec(c);
+          c->solve_key = key;
+          if (st) return st;
+          r.spec = true;
+          r.spec_dt = dt;
         }
         int sdone = 0;
         st = mg_poll(c, block, &sdone);
@@ -882,7 +924,7 @@ static int rk_advance(qdeq_ctx *c, bool block, int *done) {
             if (!stage_graphs(c)) launch_update_stage(c, r.s);
             r.staged = true;
           }
-        } else {
+        } else if (!r.spec) {
           launch_fault_update(c, r.s);
         }
         if (c->rk_ready && !r.final_stage) {
@@ -933,12 +975,7 @@ static int rk_advance(qdeq_ctx *c, bool block, int *done) {
         break;
       }
       case RK_STEP_FINISH: {
-        double newDeltaT;  // odeSolver.cpp:1621-1632
-        if (c->totErr != 0.0)
-          newDeltaT = qdeq_compute_step_size(c->it, c->dt, p.mindt, p.maxdt,
-                                             c->totErr, p.totTol, c->errA);
-        else
-          newDeltaT = c->dt;
+        const double newDeltaT = step_finish_dt(c);
         c->errA[1] = c->errA[0];
         c->errA[0] = c->totErr;
         c->dt = newDeltaT;
@@ -993,10 +1030,12 @@ int qdeq_rk43_step(qdeq_ctx *c, int nsteps, qdeq_step_info *info) {
   return QDEQ_OK;
 }
 
-int qdeq_launch_ahead_stats(qdeq_ctx *c, long long *taken, long long *abandoned) {
+int qdeq_launch_ahead_stats(qdeq_ctx *c, long long *taken, long long *abandoned,
+                            long long *taken_next_step) {
   if (!c) return set_error(c, QDEQ_ERR_ARG, "null argument");
   if (taken) *taken = c->spec_taken;
   if (abandoned) *abandoned = c->spec_abandoned;
+  if (taken_next_step) *taken_next_step = c->spec_taken_next;
   return QDEQ_OK;
 }
 

@@ -277,6 +277,33 @@ __global__ void k_update_stage(double *u, int nk, int K, int P, UpdateArgs ua,
              sa.FV, sa.Fd, sa.Ys, sa.Yp, u, K, P);
 }
 
+// The fault update after an accepted step's final stage-1 solve, then the
+// next attempt's zeroed stage rates and its stage 2 state and boundary data,
+// queued behind that solve: it runs only if the solve ended converged
+// (spec[2]).  The attempt's t and dt come by value (thread 0 also stores
+// them for the later stages), so no thread reads what another one writes.
+__global__ void k_update_next(double *u, int nk, int nz, int K, int P, UpdateArgs ua,
+                              StageArgs sa, double t, double dt, double *tdt,
+                              const int *spec) {
+  if (spec[2] != 1) return;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k == 0) {
+    tdt[0] = t;
+    tdt[1] = dt;
+  }
+  if (k >= nz) return;
+  for (int s = 2; s <= 6; s++) {
+    sa.FV[s][k] = 0.0;
+    sa.Fd[s][k] = 0.0;
+  }
+  if (k < 1 || k > nk) return;
+  update_node(k, u, P, ua.mu_f, ua.tau_n0, ua.Yp, ua.a, ua.b, ua.eta, ua.V0, ua.f0,
+              ua.L, ua.c0, ua.c1, ua.c2, ua.c3, ua.Ts, ua.Tn, ua.V, ua.dpsi,
+              ua.flags);
+  stage_node(k, 2, dt, t, sa.ratio, sa.Vini, sa.slip, sa.psi, sa.ss, sa.FV, sa.Fd,
+             sa.Ys, sa.Yp, u, K, P);
+}
+
 __global__ void k_zero_rates(double *const *FV, double *const *Fd, int nz,
                              double *tdt, double t, double dt) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -479,6 +506,19 @@ void launch_update_stage(qdeq_ctx *c, int s, const int *spec) {
                      f.slip_shift, dev_FV(c), dev_Fd(c), f.Ys[s + 1], f.Yp[s + 1]};
   hipLaunchKernelGGL(k_update_stage, nodes_grid(c->cfg.nk), dim3(FT), 0, c->stream,
                      c->d[0].u, c->cfg.nk, c->lv[0].K, c->lv[0].P, ua, sa, spec);
+}
+
+void launch_update_next(qdeq_ctx *c, double t, double dt, const int *spec) {
+  FaultBufs &f = c->fb;
+  const UpdateArgs ua{f.mu_f, f.tau_n0, f.Yp[1], f.a, f.b, f.eta, c->cfg.V0,
+                      c->cfg.f0, c->cfg.L, c->c_stress[0], c->c_stress[1],
+                      c->c_stress[2], c->c_stress[3], f.Ts, f.Tn, f.FV[1], f.Fd[1],
+                      c->flags};
+  const StageArgs sa{2, c->scal + SC_T, kC[2], c->cfg.Vini, f.slip, f.psi,
+                     f.slip_shift, dev_FV(c), dev_Fd(c), f.Ys[2], f.Yp[2]};
+  hipLaunchKernelGGL(k_update_next, nodes_grid(c->nz), dim3(FT), 0, c->stream,
+                     c->d[0].u, c->cfg.nk, c->nz, c->lv[0].K, c->lv[0].P, ua, sa, t,
+                     dt, c->scal + SC_T, spec);
 }
 
 void launch_zero_stage_rates(qdeq_ctx *c, double t, double dt) {

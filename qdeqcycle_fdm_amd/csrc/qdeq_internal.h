@@ -98,6 +98,8 @@ struct RkMachine {
   // the next stage's fault update, start and first batch are queued behind
   // stage s's solve (qdeq_ctx::mg_next), to be taken over if it converges
   bool spec = false;
+  // for the final stage 1: the next attempt's dt the queued stage 2 used
+  double spec_dt = 0.0;
   hipEvent_t ev = nullptr;   // completion of the last host readback
 };
 
@@ -149,6 +151,7 @@ struct qdeq_ctx {
   bool spec_launch = false;
   bool spec_off = false;       // QDEQ_SPECULATE=0
   long long spec_taken = 0, spec_abandoned = 0;
+  long long spec_taken_next = 0;  // of them, stage 2 of the next step
   bool stage_graphs_off = false;  // QDEQ_STAGE_GRAPHS=0 (see stage_graphs)
   int last_it = 2;             // V-cycles of the last solve (first batch size)
   // RK43 solves: the stage's key (stage s, +8 for the solve after an
@@ -435,6 +438,7 @@ void launch_update_stage(qdeq_ctx *c, int s, const int *spec = nullptr);
 void launch_fault_update(qdeq_ctx *c, int s);
 // zeroes the stage rates of a new attempt and sets its t, dt on the device
 void launch_zero_stage_rates(qdeq_ctx *c, double t, double dt);
+void launch_update_next(qdeq_ctx *c, double t, double dt, const int *spec);
 void launch_combine(qdeq_ctx *c);
 // accepts y4 and sets the device time to t_new
 void launch_commit(qdeq_ctx *c, double t_new);

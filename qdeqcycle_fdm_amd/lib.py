@@ -231,7 +231,7 @@ def load(path: str = LIB_PATH):
     L.qdeq_guess_infer.argtypes = [_vp, _fp, _fp]
     L.qdeq_projection_enable.argtypes = [_vp, ctypes.c_int]
     L.qdeq_projection_stats.argtypes = [_vp, _llp, _llp, _dp, _ip]
-    L.qdeq_launch_ahead_stats.argtypes = [_vp, _llp, _llp]
+    L.qdeq_launch_ahead_stats.argtypes = [_vp, _llp, _llp, _llp]
     L.qdeq_projection_fit.argtypes = [_vp, _dp, _dp, ctypes.c_int, _dp]
     L.qdeq_projection_inject.argtypes = [_vp, ctypes.c_double]
     for name in EXPORTS:
@@ -745,10 +745,13 @@ Engine.projection_stats = _engine_projection_stats
 
 def _engine_launch_ahead_stats(self) -> dict:
     """Solves launched ahead behind the previous stage's solve and taken
-    over, or abandoned (the previous solve needed more V-cycles)."""
-    t, a = ctypes.c_longlong(0), ctypes.c_longlong(0)
-    self._check(load().qdeq_launch_ahead_stats(self.h, ctypes.byref(t), ctypes.byref(a)))
-    return dict(taken=t.value, abandoned=a.value)
+    over, or abandoned (the previous solve needed more V-cycles); of those
+    taken, next_step counts the next step's stage 2 queued behind an
+    accepted step's final stage 1."""
+    t, a, n = ctypes.c_longlong(0), ctypes.c_longlong(0), ctypes.c_longlong(0)
+    self._check(load().qdeq_launch_ahead_stats(self.h, ctypes.byref(t), ctypes.byref(a),
+                                               ctypes.byref(n)))
+    return dict(taken=t.value, abandoned=a.value, next_step=n.value)
 
 
 Engine.launch_ahead_stats = _engine_launch_ahead_stats

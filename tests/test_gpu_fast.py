@@ -421,6 +421,61 @@ def test_stages_launched_ahead_equal_waiting_runs(n, levels, timed, monkeypatch)
         assert a["info"][key] == b["info"][key], key
     for k in ("slip", "psi", "V", "U"):
         np.testing.assert_array_equal(a[k], b[k])
-    assert sb == {"taken": 0, "abandoned": 0}
+    assert sb == {"taken": 0, "abandoned": 0, "next_step": 0}
     assert sa["taken"] > 0.5 * a["info"]["solves"]
     assert sa["abandoned"] > 0
+    # the next step's stage 2 behind most accepted steps' final stage 1
+    assert sa["next_step"] > 0.3 * 120
+
+
+def test_next_step_launched_ahead_ends_at_tmax_and_stays_off_between_records(
+        monkeypatch):
+    """Stage 2 of the next step is queued behind an accepted step's final
+    stage 1 only when that step runs within the same call and nothing is
+    recorded between the steps.  Bit for bit against QDEQ_SPECULATE=0: a run
+    whose last step is clipped to tmax inside one call (the queued stage used
+    the clipped dt), a run one step per call and a run with events on (both
+    without any stage queued across a step)."""
+    n, levels = 129, 4
+    pb = config.make_bp1_problem(n, basin=True, levels=levels,
+                                 tol_mode=config.TOL_REL, tol=1e-10)
+    me = oracle.mu_equi(pb.mu)
+    monkeypatch.setenv("QDEQ_SPECULATE", "0")
+    with lib.Engine(pb, me, **PROD) as eng:
+        eng.set_tiles(True, 64, 16)
+        eng.rk43_setup()
+        t30 = eng.rk43_step(30)["t"]
+        dt30 = eng.rk43_step(0)["dt"]
+    tmax = t30 + 0.37 * dt30  # step 31 clipped to end at tmax
+
+    def run(spec, how):
+        if spec:
+            monkeypatch.delenv("QDEQ_SPECULATE", raising=False)
+        else:
+            monkeypatch.setenv("QDEQ_SPECULATE", "0")
+        kw = dict(PROD, tmax=tmax) if how == "tmax" else dict(PROD)
+        with lib.Engine(pb, me, **kw) as eng:
+            eng.set_tiles(True, 64, 16)
+            eng.rk43_setup()
+            if how == "tmax":
+                eng.rk43_step(50)
+            elif how == "single":
+                for _ in range(25):
+                    eng.rk43_step(1)
+            else:
+                eng.events_enable(1e-3)
+                eng.rk43_step(25)
+            return eng.state(), eng.launch_ahead_stats()
+
+    for how in ("tmax", "single", "events"):
+        (a, sa), (b, sb) = run(True, how), run(False, how)
+        for key in ("t", "dt", "vcycles", "solves", "attempts"):
+            assert a["info"][key] == b["info"][key], (how, key)
+        for k in ("slip", "psi", "V", "U"):
+            np.testing.assert_array_equal(a[k], b[k], err_msg=how)
+        assert sb["taken"] == 0
+        if how == "tmax":
+            assert a["info"]["t"] >= tmax
+            assert sa["next_step"] > 10
+        else:
+            assert sa["next_step"] == 0 and sa["taken"] > 0
