@@ -237,9 +237,172 @@ __device__ inline void tile_half_sweep(Tile<T> &t, int h, int colour, int N,
   }
 }
 
+// The sweeps with each thread holding fixed 2x2 quads of the tile (local
+// rows r, r+1 and columns c, c+1, r and c even) in registers: their values,
+// coefficients and right-hand sides.  The tile's origin (j0, k0) has even
+// parity (even tile sizes, halo on both sides), so in every quad (r, c) and
+// (r+1, c+1) are colour 0 and the other two colour 1, for every thread alike.
+// A point's neighbours are two of its own quad (registers) and two outside
+// it (LDS): two LDS reads per update instead of nine.  Every update is also
+// stored to the tile, which stays current for the other threads and for
+// what follows the sweeps.  Values, coefficients and expressions are those
+// of tile_half_sweep, so the results are the same bits.
+template <typename T, bool HAS_F>
+struct Quads1 {
+  T u00, u01, u10, u11;
+  T x0, x1, x2;  // scx row r, columns c .. c+2
+  T y0, y1, y2;  // scx row r+1
+  T z0, z1;      // scz row r (south faces of row r)
+  T w0, w1;      // scz row r+1
+  T v0, v1;      // scz row r+2
+  T f00, f01, f10, f11;  // DH^2 f (HAS_F)
+  int r, c;
+  bool valid;
+};
+
+// Point P of quad q in the half-sweep of colour COL: P = 0 the upper point
+// of the colour (row r), P = 1 the lower (row r+1).
+template <typename T, bool HAS_F, bool EXACT, int COL, int P>
+__device__ inline void quad_point_read(const Tile<T> &t, const Quads1<T, HAS_F> &q,
+                                       int h, int N, int K, bool &ok, int &at,
+                                       T &hout, T &vout) {
+  const bool top = P == 0;
+  const bool left = (COL == 0) == (P == 0);
+  const int lr = q.r + P;
+  const int lc = q.c + (left ? 0 : 1);
+  const int j = t.j0 + lr, k = t.k0 + lc;
+  ok = q.valid && lr >= h && lr < t.R - h && lc >= h && lc < t.C - h && j >= 0 &&
+       j < N && k >= 1 && k <= K - 2;
+  at = lr * t.LS + lc;
+  const int ih = ok ? at + (left ? -1 : 1) : t.LS + 1;
+  const int iv = ok ? at + (top ? -t.LS : t.LS) : t.LS + 1;
+  hout = t.su[ih];
+  vout = t.su[iv];
+}
+
+template <typename T, bool HAS_F, bool EXACT, int COL, int P>
+__device__ inline T quad_point_update(const Tile<T> &t, const Quads1<T, HAS_F> &q,
+                                      int N, T hout, T vout, T relax) {
+  const int j = t.j0 + q.r + P;
+  T m1, m2, m3, m4, uW, uE, uC, vin, fv;
+  if (COL == 0 && P == 0) {  // (r, c)
+    m1 = q.x0; m2 = q.x1; m3 = q.z0; m4 = q.w0;
+    uW = hout; uE = q.u01; uC = q.u00; vin = q.u10; fv = q.f00;
+  } else if (COL == 0) {     // (r+1, c+1)
+    m1 = q.y1; m2 = q.y2; m3 = q.w1; m4 = q.v1;
+    uW = q.u10; uE = hout; uC = q.u11; vin = q.u01; fv = q.f11;
+  } else if (P == 0) {       // (r, c+1)
+    m1 = q.x1; m2 = q.x2; m3 = q.z1; m4 = q.w1;
+    uW = q.u00; uE = hout; uC = q.u01; vin = q.u11; fv = q.f01;
+  } else {                   // (r+1, c)
+    m1 = q.y0; m2 = q.y1; m3 = q.w0; m4 = q.v0;
+    uW = hout; uE = q.u11; uC = q.u10; vin = q.u00; fv = q.f10;
+  }
+  // south/north neighbours; rows 0 and N-1 of the grid reflect (the
+  // neighbour across the boundary is the one on the other side)
+  T uS, uN;
+  if (P == 0) {
+    uS = (j == 0) ? vin : vout;
+    uN = (j == N - 1) ? vout : vin;
+  } else {
+    uS = (j == 0) ? vout : vin;
+    uN = (j == N - 1) ? vin : vout;
+  }
+  return rbgs_update<T, EXACT>(m1, m2, m3, m4, uW, uE, uS, uN, uC,
+                               HAS_F ? fv : T(0), HAS_F, relax);
+}
+
+// One quad at a time: its two points' reads, updates, then stores.
+template <typename T, bool HAS_F, bool EXACT, int COL, int S>
+__device__ inline void quad_half_sweep(Tile<T> &t, Quads1<T, HAS_F> (&q)[S], int h,
+                                       int N, int K, T relax) {
+#pragma unroll
+  for (int i = 0; i < S; i++) {
+    bool ok0, ok1;
+    int at0, at1;
+    T h0, v0, h1, v1;
+    quad_point_read<T, HAS_F, EXACT, COL, 0>(t, q[i], h, N, K, ok0, at0, h0, v0);
+    quad_point_read<T, HAS_F, EXACT, COL, 1>(t, q[i], h, N, K, ok1, at1, h1, v1);
+    const T n0 = quad_point_update<T, HAS_F, EXACT, COL, 0>(t, q[i], N, h0, v0, relax);
+    const T n1 = quad_point_update<T, HAS_F, EXACT, COL, 1>(t, q[i], N, h1, v1, relax);
+    if (ok0) {
+      t.su[at0] = n0;
+      if (COL == 0) q[i].u00 = n0; else q[i].u01 = n0;
+    }
+    if (ok1) {
+      t.su[at1] = n1;
+      if (COL == 0) q[i].u11 = n1; else q[i].u10 = n1;
+    }
+  }
+}
+
+// Whether the sweeps run on register quads, and with how many per thread
+// (0: the point walk): two or more sweeps (loading the registers costs
+// about one half-sweep of LDS reads), up to two quads per thread (one with
+// a right-hand side in FP64).
+template <typename T, bool HAS_F>
+__device__ inline int quads_per_thread(const Tile<T> &t, int nu) {
+  const int nthr = blockDim.x * blockDim.y;
+  const int nq = (t.R >> 1) * (t.C >> 1);
+  if (((t.R | t.C) & 1) != 0 || nu < 2) return 0;
+  if (nq <= nthr) return 1;
+  if (nq <= 2 * nthr && (!HAS_F || sizeof(T) == 4)) return 2;
+  return 0;
+}
+
+template <typename T, bool HAS_F, bool EXACT, int S>
+__device__ inline void tile_sweeps_quads(Tile<T> &t, int nu, int N, int K, T DH2,
+                                         T relax) {
+  const int nthr = blockDim.x * blockDim.y;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  const int qc = t.C >> 1, nq = (t.R >> 1) * qc;
+  const int LS = t.LS;
+  Quads1<T, HAS_F> q[S];
+#pragma unroll
+  for (int i = 0; i < S; i++) {
+    const int id = tid + i * nthr;
+    q[i].valid = id < nq;
+    const int a = q[i].valid ? id / qc : 0;
+    const int b = q[i].valid ? id - a * qc : 0;
+    q[i].r = 2 * a;
+    q[i].c = 2 * b;
+    // rows r .. r+2 of scz and column c+2 of scx stay inside the tile's
+    // LDS (scz has R+1 rows; column C of a row is the next row's first)
+    const int s = q[i].r * LS + q[i].c;
+    q[i].u00 = t.su[s]; q[i].u01 = t.su[s + 1];
+    q[i].u10 = t.su[s + LS]; q[i].u11 = t.su[s + LS + 1];
+    q[i].x0 = t.scx[s]; q[i].x1 = t.scx[s + 1]; q[i].x2 = t.scx[s + 2];
+    q[i].y0 = t.scx[s + LS]; q[i].y1 = t.scx[s + LS + 1]; q[i].y2 = t.scx[s + LS + 2];
+    q[i].z0 = t.scz[s]; q[i].z1 = t.scz[s + 1];
+    q[i].w0 = t.scz[s + LS]; q[i].w1 = t.scz[s + LS + 1];
+    q[i].v0 = t.scz[s + 2 * LS]; q[i].v1 = t.scz[s + 2 * LS + 1];
+    if (HAS_F) {
+      q[i].f00 = DH2 * t.sf[s]; q[i].f01 = DH2 * t.sf[s + 1];
+      q[i].f10 = DH2 * t.sf[s + LS]; q[i].f11 = DH2 * t.sf[s + LS + 1];
+    } else {
+      q[i].f00 = q[i].f01 = q[i].f10 = q[i].f11 = T(0);
+    }
+  }
+  for (int s = 0; s < 2 * nu; s += 2) {
+    quad_half_sweep<T, HAS_F, EXACT, 0, S>(t, q, s + 1, N, K, relax);
+    __syncthreads();
+    quad_half_sweep<T, HAS_F, EXACT, 1, S>(t, q, s + 2, N, K, relax);
+    __syncthreads();
+  }
+}
+
 template <typename T, bool HAS_F, bool EXACT>
 __device__ inline void tile_sweeps(Tile<T> &t, int nu, int N, int K, T DH2,
                                    T relax) {
+  const int qs = quads_per_thread<T, HAS_F>(t, nu);
+  if (qs == 1) {
+    tile_sweeps_quads<T, HAS_F, EXACT, 1>(t, nu, N, K, DH2, relax);
+    return;
+  }
+  if (qs == 2) {
+    tile_sweeps_quads<T, HAS_F, EXACT, 2>(t, nu, N, K, DH2, relax);
+    return;
+  }
   for (int s = 0; s < 2 * nu; s++) {
     tile_half_sweep<T, HAS_F, EXACT>(t, s + 1, s & 1, N, K, DH2, relax);
     __syncthreads();
