@@ -687,29 +687,36 @@ __global__ __launch_bounds__(TILE_THREADS) void k_up(
   tile_load_up<T, HAS_F, FULL_TOP>(t, uin, f, cx, cz, N, K, P, e, Nc, Kc, Pc);
   __syncthreads();
   tile_sweeps<T, HAS_F, EXACT>(t, nu, N, K, DH2, relax);
-  double acc = 0.0;
+  if (MODE == UP_NORM) {
+    // the residual-norm partial of the tile's interior first: the block
+    // sum's barrier then has no global stores of this kernel to wait for
+    double acc = 0.0;
+    for (FlatWalk w(TX); w.r < TY; w.next()) {
+      const int lr = H + w.r, lc = H + w.c;
+      const int j = t.j0 + lr;
+      const int k = t.k0 + lc;
+      if (j < N && k >= 1 && k <= K - 2) {
+        const double r = (double)tile_residual<T, HAS_F>(t, lr, lc, j, N, DH2);
+        acc += r * r;
+      }
+    }
+    __shared__ double red[16];
+    const double s = block_sum1(acc, red);
+    if (threadIdx.x == 0 && threadIdx.y == 0)
+      partial[tile_id()] = s;
+  }
   for (FlatWalk w(TX); w.r < TY; w.next()) {
     const int lr = H + w.r, lc = H + w.c;
     const int j = t.j0 + lr;
     const int k = t.k0 + lc;
     if (j < N && k < K) {
       const size_t g = (size_t)j * P + k;
-      if (MODE == UP_NORM && k >= 1 && k <= K - 2) {
-        const double r = (double)tile_residual<T, HAS_F>(t, lr, lc, j, N, DH2);
-        acc += r * r;
-      }
       if (MODE == UP_ADD64) {
         if (k >= 1 && k <= K - 2) u64[g] += (double)t.su[lr * t.LS + lc];
       } else {
         uout[g] = t.su[lr * t.LS + lc];
       }
     }
-  }
-  if (MODE == UP_NORM) {
-    __shared__ double red[16];
-    const double s = block_sum1(acc, red);
-    if (threadIdx.x == 0 && threadIdx.y == 0)
-      partial[tile_id()] = s;
   }
 }
 

@@ -351,10 +351,34 @@ int build_coarse_inverse(qdeq_ctx *c);
 // Sum over the 64 lanes of a wavefront by a butterfly over lane distances
 // 32, 16, ..., 1: a fixed order of additions, and every lane ends with the
 // same bits (each pair adds the same two values).
+// v of the lane that DPP control CTRL selects (both 32-bit halves moved).
+template <int CTRL>
+__device__ inline double dpp_move(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the 64 lanes of a wavefront (every lane active), returned to
+// every lane: lane moves within rows of 16 (xor 1 and xor 2 in each quad,
+// then the half-row and row mirrors; the two operands of each addition
+// differ between lanes only in order, so every lane of a row holds the same
+// bits), then the four row sums added in row order.
 __device__ inline double wave_sum(double v) {
+  v = v + dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]
+  v = v + dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]
+  v = v + dpp_move<0x141>(v);  // row_half_mirror
+  v = v + dpp_move<0x140>(v);  // row_mirror
+  const long long b = __double_as_longlong(v);
+  double r[4];
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  return v;
+  for (int i = 0; i < 4; i++) {
+    const int lo = __builtin_amdgcn_readlane((int)b, 16 * i);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 16 * i);
+    r[i] = __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  }
+  return ((r[0] + r[1]) + r[2]) + r[3];
 }
 
 // Sums of NV values over the workgroup, returned to every thread: wavefront
