@@ -256,9 +256,31 @@ struct Quads1 {
   T w0, w1;      // scz row r+1
   T v0, v1;      // scz row r+2
   T f00, f01, f10, f11;  // DH^2 f (HAS_F)
+  // per-point relaxation factor relax / (m1 + m2 + m3 + m4) of the exact
+  // update, or the coefficient sum m1 + m2 + m3 + m4 of the plain one, when
+  // the quads keep it (PRE)
+  T a00, a01, a10, a11;
   int r, c;
   bool valid;
 };
+
+// The update of rbgs_update with the per-point quantity a formed once:
+// relax / (m1 + m2 + m3 + m4) (EXACT) or m1 + m2 + m3 + m4 (plain), the
+// same operations in the same order, so the same bits.
+template <typename T, bool EXACT>
+__device__ inline T rbgs_update_pre(T a, T m1, T m2, T m3, T m4, T uW, T uE, T uS,
+                                    T uN, T uC, T fDH2, bool has_f, T relax) {
+  T num = m1 * uW + m2 * uE + m3 * uS + m4 * uN;
+  if (has_f) num = num - fDH2;
+  if (EXACT) return a * num + (T(1) - relax) * uC;
+  const T unew = num / a;
+  return (relax == T(1)) ? unew : uC + relax * (unew - uC);
+}
+
+template <typename T, bool EXACT>
+__device__ inline T rbgs_factor(T m1, T m2, T m3, T m4, T relax) {
+  return EXACT ? relax / (m1 + m2 + m3 + m4) : m1 + m2 + m3 + m4;
+}
 
 // Point P of quad q in the half-sweep of colour COL: P = 0 the upper point
 // of the colour (row r), P = 1 the lower (row r+1).
@@ -280,23 +302,23 @@ __device__ inline void quad_point_read(const Tile<T> &t, const Quads1<T, HAS_F> 
   vout = t.su[iv];
 }
 
-template <typename T, bool HAS_F, bool EXACT, int COL, int P>
+template <typename T, bool HAS_F, bool EXACT, int COL, int P, bool PRE>
 __device__ inline T quad_point_update(const Tile<T> &t, const Quads1<T, HAS_F> &q,
                                       int N, T hout, T vout, T relax) {
   const int j = t.j0 + q.r + P;
-  T m1, m2, m3, m4, uW, uE, uC, vin, fv;
+  T m1, m2, m3, m4, uW, uE, uC, vin, fv, a;
   if (COL == 0 && P == 0) {  // (r, c)
     m1 = q.x0; m2 = q.x1; m3 = q.z0; m4 = q.w0;
-    uW = hout; uE = q.u01; uC = q.u00; vin = q.u10; fv = q.f00;
+    uW = hout; uE = q.u01; uC = q.u00; vin = q.u10; fv = q.f00; a = q.a00;
   } else if (COL == 0) {     // (r+1, c+1)
     m1 = q.y1; m2 = q.y2; m3 = q.w1; m4 = q.v1;
-    uW = q.u10; uE = hout; uC = q.u11; vin = q.u01; fv = q.f11;
+    uW = q.u10; uE = hout; uC = q.u11; vin = q.u01; fv = q.f11; a = q.a11;
   } else if (P == 0) {       // (r, c+1)
     m1 = q.x1; m2 = q.x2; m3 = q.z1; m4 = q.w1;
-    uW = q.u00; uE = hout; uC = q.u01; vin = q.u11; fv = q.f01;
+    uW = q.u00; uE = hout; uC = q.u01; vin = q.u11; fv = q.f01; a = q.a01;
   } else {                   // (r+1, c)
     m1 = q.y0; m2 = q.y1; m3 = q.w0; m4 = q.v0;
-    uW = hout; uE = q.u11; uC = q.u10; vin = q.u00; fv = q.f10;
+    uW = hout; uE = q.u11; uC = q.u10; vin = q.u00; fv = q.f10; a = q.a10;
   }
   // south/north neighbours; rows 0 and N-1 of the grid reflect (the
   // neighbour across the boundary is the one on the other side)
@@ -308,12 +330,15 @@ __device__ inline T quad_point_update(const Tile<T> &t, const Quads1<T, HAS_F> &
     uS = (j == 0) ? vout : vin;
     uN = (j == N - 1) ? vin : vout;
   }
+  if (PRE)
+    return rbgs_update_pre<T, EXACT>(a, m1, m2, m3, m4, uW, uE, uS, uN, uC,
+                                     HAS_F ? fv : T(0), HAS_F, relax);
   return rbgs_update<T, EXACT>(m1, m2, m3, m4, uW, uE, uS, uN, uC,
                                HAS_F ? fv : T(0), HAS_F, relax);
 }
 
 // One quad at a time: its two points' reads, updates, then stores.
-template <typename T, bool HAS_F, bool EXACT, int COL, int S>
+template <typename T, bool HAS_F, bool EXACT, int COL, int S, bool PRE>
 __device__ inline void quad_half_sweep(Tile<T> &t, Quads1<T, HAS_F> (&q)[S], int h,
                                        int N, int K, T relax) {
 #pragma unroll
@@ -323,8 +348,10 @@ __device__ inline void quad_half_sweep(Tile<T> &t, Quads1<T, HAS_F> (&q)[S], int
     T h0, v0, h1, v1;
     quad_point_read<T, HAS_F, EXACT, COL, 0>(t, q[i], h, N, K, ok0, at0, h0, v0);
     quad_point_read<T, HAS_F, EXACT, COL, 1>(t, q[i], h, N, K, ok1, at1, h1, v1);
-    const T n0 = quad_point_update<T, HAS_F, EXACT, COL, 0>(t, q[i], N, h0, v0, relax);
-    const T n1 = quad_point_update<T, HAS_F, EXACT, COL, 1>(t, q[i], N, h1, v1, relax);
+    const T n0 =
+        quad_point_update<T, HAS_F, EXACT, COL, 0, PRE>(t, q[i], N, h0, v0, relax);
+    const T n1 =
+        quad_point_update<T, HAS_F, EXACT, COL, 1, PRE>(t, q[i], N, h1, v1, relax);
     if (ok0) {
       t.su[at0] = n0;
       if (COL == 0) q[i].u00 = n0; else q[i].u01 = n0;
@@ -353,6 +380,9 @@ __device__ inline int quads_per_thread(const Tile<T> &t, int nu) {
 template <typename T, bool HAS_F, bool EXACT, int S>
 __device__ inline void tile_sweeps_quads(Tile<T> &t, int nu, int N, int K, T DH2,
                                          T relax) {
+  // the per-point factor in registers too where they allow it: one quad
+  // per thread, or FP32 values (two FP64 quads already fill the registers)
+  constexpr bool PRE = S == 1 || sizeof(T) == 4;
   const int nthr = blockDim.x * blockDim.y;
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
   const int qc = t.C >> 1, nq = (t.R >> 1) * qc;
@@ -382,11 +412,19 @@ __device__ inline void tile_sweeps_quads(Tile<T> &t, int nu, int N, int K, T DH2
     } else {
       q[i].f00 = q[i].f01 = q[i].f10 = q[i].f11 = T(0);
     }
+    if (PRE) {
+      q[i].a00 = rbgs_factor<T, EXACT>(q[i].x0, q[i].x1, q[i].z0, q[i].w0, relax);
+      q[i].a01 = rbgs_factor<T, EXACT>(q[i].x1, q[i].x2, q[i].z1, q[i].w1, relax);
+      q[i].a10 = rbgs_factor<T, EXACT>(q[i].y0, q[i].y1, q[i].w0, q[i].v0, relax);
+      q[i].a11 = rbgs_factor<T, EXACT>(q[i].y1, q[i].y2, q[i].w1, q[i].v1, relax);
+    } else {
+      q[i].a00 = q[i].a01 = q[i].a10 = q[i].a11 = T(0);
+    }
   }
   for (int s = 0; s < 2 * nu; s += 2) {
-    quad_half_sweep<T, HAS_F, EXACT, 0, S>(t, q, s + 1, N, K, relax);
+    quad_half_sweep<T, HAS_F, EXACT, 0, S, PRE>(t, q, s + 1, N, K, relax);
     __syncthreads();
-    quad_half_sweep<T, HAS_F, EXACT, 1, S>(t, q, s + 2, N, K, relax);
+    quad_half_sweep<T, HAS_F, EXACT, 1, S, PRE>(t, q, s + 2, N, K, relax);
     __syncthreads();
   }
 }
@@ -774,6 +812,45 @@ __device__ inline T *tail_end(T *base, const TailArgs &a) {
 // Sweeps on one tail level.  With one_wave, the first wavefront alone runs
 // them (small coarsest grids), ordering its own LDS traffic between
 // half-sweeps with wavefront fences instead of workgroup barriers.
+// One point of colour COL of a thread that holds its coefficients, DH^2 f
+// and relaxation factor in registers.
+template <typename T>
+struct TailPoint {
+  T m1, m2, m3, m4, a, fv;
+  int c, cS, cN;
+  bool ok;
+};
+
+template <typename T, bool EXACT>
+__device__ inline TailPoint<T> tail_point(const TailBufs<T> &b, int N, int K, T DH2,
+                                          T relax, bool act, int j, int q, int col) {
+  TailPoint<T> p;
+  const int k = 2 * q + ((j + col) & 1);
+  p.ok = act && k >= 1 && k <= K - 2;
+  const int kk = p.ok ? k : 1;  // a point that stays put reads in-range values
+  const int jS = (j == 0) ? 1 : j - 1;
+  const int jN = (j == N - 1) ? N - 2 : j + 1;
+  p.c = j * K + kk;
+  p.cS = jS * K + kk;
+  p.cN = jN * K + kk;
+  p.m1 = b.cx[p.c];
+  p.m2 = b.cx[p.c + 1];
+  p.m3 = b.cz[p.c];
+  p.m4 = b.cz[p.c + K];
+  p.a = rbgs_factor<T, EXACT>(p.m1, p.m2, p.m3, p.m4, relax);
+  p.fv = DH2 * b.f[p.c];
+  return p;
+}
+
+template <typename T, bool EXACT>
+__device__ inline void tail_point_update(const TailBufs<T> &b, const TailPoint<T> &p,
+                                         T relax) {
+  const T v = rbgs_update_pre<T, EXACT>(p.a, p.m1, p.m2, p.m3, p.m4, b.u[p.c - 1],
+                                        b.u[p.c + 1], b.u[p.cS], b.u[p.cN], b.u[p.c],
+                                        p.fv, true, relax);
+  if (p.ok) b.u[p.c] = v;
+}
+
 template <typename T, bool EXACT>
 __device__ inline void tail_smooth(const TailBufs<T> &b, int N, int K, T DH2,
                                    T relax, int sweeps, bool one_wave) {
@@ -781,6 +858,22 @@ __device__ inline void tail_smooth(const TailBufs<T> &b, int N, int K, T DH2,
   const int tid = threadIdx.x;
   const int nthr = one_wave ? 64 : (int)blockDim.x;
   if (one_wave && tid >= 64) return;
+  if (N * half <= nthr && !one_wave) {
+    // one point of each colour per thread: its coefficients, right-hand
+    // side and factor read once into registers (f does not change while the
+    // level is smoothed), the same update expression
+    const bool act = tid < N * half;
+    const int j = act ? tid / half : 0, q = act ? tid - j * half : 0;
+    const TailPoint<T> p0 = tail_point<T, EXACT>(b, N, K, DH2, relax, act, j, q, 0);
+    const TailPoint<T> p1 = tail_point<T, EXACT>(b, N, K, DH2, relax, act, j, q, 1);
+    for (int s = 0; s < sweeps; s++) {
+      tail_point_update<T, EXACT>(b, p0, relax);
+      __syncthreads();
+      tail_point_update<T, EXACT>(b, p1, relax);
+      __syncthreads();
+    }
+    return;
+  }
   for (int s = 0; s < 2 * sweeps; s++) {
     const int colour = s & 1;
     for (int idx = tid; idx < N * half; idx += nthr) {
