@@ -348,9 +348,6 @@ __device__ inline void coarse_matvec(const float *__restrict__ minvT, const T *g
 
 int build_coarse_inverse(qdeq_ctx *c);
 
-// Sum over the 64 lanes of a wavefront by a butterfly over lane distances
-// 32, 16, ..., 1: a fixed order of additions, and every lane ends with the
-// same bits (each pair adds the same two values).
 // v of the lane that DPP control CTRL selects (both 32-bit halves moved).
 template <int CTRL>
 __device__ inline double dpp_move(double v) {
