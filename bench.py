@@ -30,10 +30,13 @@ Side measurements, each of a fixed size that does not depend on K:
   fast64's rate over the same steps (speedup_vs_ref64), and ir32;
 * an ensemble leg: --ensemble-leg members of the seeded 64-member draw,
   --ensemble-leg-steps steps each, alone and stepped together;
-* the CPU baseline: the project's FP64 port of the solver (oracle/) restarted
-  from the GPU's state at the end of the timed region, on one host core for
-  about --cpu-seconds, then the same steps on every core; the GPU then runs
-  the same steps and the three final states are compared bit for bit.
+* the CPU baseline: the project's FP64 port of the solver (oracle/) over the
+  workload's first 20 accepted steps from t = 0 (fewer if one host core
+  passes twice --cpu-seconds), on one core, then the same steps on every
+  core; the GPU then runs the same steps from t = 0 and the three final
+  states are compared bit for bit;
+* with --c1, config 1 in full as well (129^2, the reference's own cycle,
+  2000 steps): GPU, one core and every core, bit for bit.
 """
 import argparse
 import json
@@ -453,44 +456,50 @@ def cpu_cores() -> int:
     return min(n, int(env)) if env.isdigit() and int(env) > 0 else n
 
 
-def cpu_baseline(pb, me, eng, seconds):
-    """The oracle restarted from the engine's state: on one thread for about
-    `seconds`, then the same steps on every core (OpenMP over the
-    multigrid's point loops); the engine then runs the same steps.  Returns
-    both CPU rates, the GPU rate on the same steps and whether the three
-    final states agree bit for bit."""
+def _attempts_per_step(solves, steps):
+    """RK43 attempts per accepted step from the solve count: one solve at
+    t = 0, five per attempt (stages 2..6) and one per accepted step (its
+    final stage 1)."""
+    return (solves - steps - 1) / 5.0 / steps if steps else None
+
+
+def cpu_baseline(pb, me, eng, seconds, steps=20):
+    """The oracle from t = 0 over the workload's first `steps` accepted steps
+    (fewer if one thread passes 2 x `seconds`): on one thread, then the same
+    steps on every core (OpenMP over the multigrid's point loops); the
+    engine, set back to t = 0, then runs the same steps.  Returns both CPU
+    rates, the GPU rate on the same steps, V-cycles per solve and attempts
+    per step, and whether the three final states agree bit for bit."""
     import numpy as np
     import oracle
-    st = eng.state()
-    info = st["info"]
     p = MODES["fast64"]
     oracle.set_libm(False)
 
-    def restart():
+    def fresh():
         sim = oracle.Sim(pb, me, cycle=1, n_pre=p["n_pre"], n_post=p["n_post"],
                          n_coarse=p["n_coarse"], relax=p["relax"])
-        sim.set_state(info["t"], info["dt"], info["it"], info["errA0"], info["errA1"],
-                      st["U"], st["slip"], st["psi"])
+        sim.init()
         return sim
 
     oracle.set_threads(1)
-    sim1 = restart()
+    sim1 = fresh()
     n = 0
     t0 = time.perf_counter()
-    while True:
+    while n < steps:
         sim1.step()
         n += 1
-        if time.perf_counter() - t0 >= seconds:
+        if time.perf_counter() - t0 >= 2 * seconds:
             break
     cpu1_s = time.perf_counter() - t0
     cores = cpu_cores()
     used = oracle.set_threads(cores)
-    simn = restart()
+    simn = fresh()
     t0 = time.perf_counter()
     for _ in range(n):
         simn.step()
     cpun_s = time.perf_counter() - t0
     oracle.set_threads(1)
+    eng.rk43_setup()
     sync()
     g0 = time.perf_counter()
     eng.rk43_step(n)
@@ -502,10 +511,60 @@ def cpu_baseline(pb, me, eng, seconds):
         ar = sim.arrays()
         same = same and all(np.array_equal(st2[k], ar[k]) for k in ("slip", "psi", "V", "U"))
         same = same and st2["info"]["t"] == sim.scalars()["t"]
+    sc = sim1.scalars()
     return dict(steps=n, cpu_seconds=cpu1_s, cpu_steps_per_s=n / cpu1_s,
                 cpu_all_seconds=cpun_s, cpu_all_steps_per_s=n / cpun_s, threads=used,
                 gpu_steps_per_s_same_steps=n / gpu_s, bitwise_equal=bool(same),
-                t_start=info["t"], t_end=st2["info"]["t"])
+                vcycles_per_solve=sc["vcycles"] / sc["solves"],
+                attempts_per_step=_attempts_per_step(sc["solves"], n),
+                t_end=st2["info"]["t"])
+
+
+def c1_leg():
+    """Config 1 in full (bp1_129: 129^2, L = 4, the reference's own V-cycle
+    with 32 sweeps, 2000 accepted steps): the engine, then the oracle on one
+    thread and on every core; rates, wall times, V-cycles per solve,
+    attempts per step and bit-equality of the three final states."""
+    import numpy as np
+    import oracle
+    from qdeqcycle_fdm_amd import config, lib
+    steps = 2000
+    pb = config.workload("bp1_129")
+    me = lib.build_mu_equi(pb.mu)
+    oracle.set_libm(False)
+    with lib.Engine(pb, me) as e1:
+        e1.rk43_setup()
+        sync()
+        g0 = time.perf_counter()
+        e1.rk43_step(steps)
+        sync()
+        gpu_s = time.perf_counter() - g0
+        gst = e1.state()
+    out = {"workload": "bp1_129", "steps": steps, "gpu_seconds": round(gpu_s, 3),
+           "gpu_steps_per_s": round(steps / gpu_s, 2)}
+    same = True
+    for label, threads in (("cpu_1_thread", 1), ("cpu_all_cores", cpu_cores())):
+        used = oracle.set_threads(threads)
+        sim = oracle.Sim(pb, me)
+        sim.init()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sim.step()
+        sec = time.perf_counter() - t0
+        sc, ar = sim.scalars(), sim.arrays()
+        same = same and all(np.array_equal(gst[k], ar[k]) for k in ("slip", "psi", "V", "U"))
+        same = same and gst["info"]["t"] == sc["t"]
+        out[label] = {"threads": used, "seconds": round(sec, 2),
+                      "steps_per_s": round(steps / sec, 3),
+                      "vcycles_per_solve": round(sc["vcycles"] / sc["solves"], 4),
+                      "attempts_per_step": round(_attempts_per_step(sc["solves"], steps), 4)}
+        log(f"config 1 {label}: {out[label]['steps_per_s']} steps/s "
+            f"({out[label]['seconds']} s on {used} threads)")
+    oracle.set_threads(1)
+    out["gpu_speedup_vs_cpu_1_thread"] = round(out["gpu_steps_per_s"] /
+                                               out["cpu_1_thread"]["steps_per_s"], 1)
+    out["bitwise_equal"] = bool(same)
+    return out
 
 
 def main():
@@ -518,9 +577,13 @@ def main():
                     help="timed steps of the ir32 mode (ref64 runs a tenth)")
     ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--c1", action="store_true",
+                    help="also run config 1 (129^2, 2000 steps) in full on the GPU "
+                         "and the CPU (several minutes of CPU time)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
-                    help="one-core CPU baseline length (the all-core run repeats "
-                         "the same steps)")
+                    help="bound of the one-core CPU baseline: it stops before its "
+                         "20 steps once twice this has passed (the all-core run "
+                         "repeats the same steps)")
     ap.add_argument("--ensemble", type=int, default=0, metavar="M",
                     help="only run the ensemble workload: M members, "
                          "--ensemble-steps steps each, alone and together")
@@ -775,16 +838,19 @@ def main():
             f"{cb['gpu_steps_per_s_same_steps']:.1f}; bitwise equal: {cb['bitwise_equal']}")
         cpu = {"value": round(cb["cpu_steps_per_s"], 4), "unit": "steps/s",
                "cores": 1, "kind": "port",
-               "sample": (f"{cb['steps']} accepted steps of basin_1025 restarted "
-                          f"from the GPU state at t={cb['t_start']:.6g} s "
-                          f"(after the timed region), same fast FP64 cycle, "
-                          f"oracle/ C port on one core"),
+               "sample": (f"the first {cb['steps']} accepted steps of basin_1025 "
+                          f"from t = 0 (to t = {cb['t_end']:.6g} s), same fast FP64 "
+                          f"cycle, oracle/ C port on one core"),
+               "vcycles_per_solve": round(cb["vcycles_per_solve"], 4),
+               "attempts_per_step": round(cb["attempts_per_step"], 4),
                "all_cores": {"value": round(cb["cpu_all_steps_per_s"], 4),
                              "cores": cb["threads"],
                              "sample": "the same steps, OpenMP over the multigrid's "
                                        "point loops"},
                "gpu_steps_per_s_same_steps": round(cb["gpu_steps_per_s_same_steps"], 2),
                "bitwise_equal_to_gpu": cb["bitwise_equal"]}
+        if args.c1:
+            cpu["config_1"] = c1_leg()
     eng.close()
 
     head = main_run
