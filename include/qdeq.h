@@ -314,6 +314,15 @@ int qdeq_projection_stats(qdeq_ctx *ctx, long long *offered,
 int qdeq_launch_ahead_stats(qdeq_ctx *ctx, long long *taken, long long *abandoned,
                             long long *taken_next_step);
 
+/* Stop tests folded into the next V-cycle so far (no reference
+ * counterpart): within a batch of V-cycles launched kernel by kernel (one
+ * simulation, FP64 tiled cycle), the residual-norm test of every V-cycle
+ * but the batch's last runs in the next V-cycle's level-0 down kernel
+ * instead of a kernel of its own; every workgroup sums the norm partials in
+ * the stop-test kernel's order, so the decisions and results are the same
+ * (QDEQ_FOLD_STOP=0 in the environment at qdeq_create turns it off). */
+int qdeq_folded_stop_tests(qdeq_ctx *ctx, long long *folded);
+
 /* Copies the current state to host arrays (any may be NULL): U (nx*nz, with
  * ghosts as ode_solve leaves them), slip, psi, V, dpsi, Ts (nz). */
 int qdeq_rk43_get_state(qdeq_ctx *ctx, qdeq_step_info *info, double *U,

@@ -265,6 +265,9 @@ int qdeq_create(const qdeq_config *cfg, const double *mu,
   // QDEQ_SPECULATE=0: every stage waits for the previous solve's end
   const char *sp = std::getenv("QDEQ_SPECULATE");
   c->spec_off = sp && sp[0] == '0';
+  // QDEQ_FOLD_STOP=0: every V-cycle of a batch has its own stop-test kernel
+  const char *fs = std::getenv("QDEQ_FOLD_STOP");
+  c->fold_off = fs && fs[0] == '0';
   *out = c;
   return QDEQ_OK;
 }
@@ -1027,6 +1030,12 @@ int qdeq_rk43_step(qdeq_ctx *c, int nsteps, qdeq_step_info *info) {
   c->rk.phase = RK_IDLE;
   if (info) return qdeq_rk43_get_state(c, info, nullptr, nullptr, nullptr,
                                        nullptr, nullptr, nullptr);
+  return QDEQ_OK;
+}
+
+int qdeq_folded_stop_tests(qdeq_ctx *c, long long *folded) {
+  if (!c || !folded) return set_error(c, QDEQ_ERR_ARG, "null argument");
+  *folded = c->folded_tests;
   return QDEQ_OK;
 }
 

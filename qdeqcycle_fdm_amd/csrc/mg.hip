@@ -581,6 +581,7 @@ static void vcycle(qdeq_ctx *c, LevelBufs<T> *B, int level, bool level0_has_f) {
 // Stop test of a V-cycle launched inside a batch (gate_ptr set), or the
 // plain norm of the initial check.
 static void launch_norm_final(qdeq_ctx *c, int nparts) {
+  if (c->gate_ptr && c->skip_gate) return;  // the next V-cycle runs this test
   if (c->gate_ptr)
     hipLaunchKernelGGL(k_norm_gate, dim3(1), dim3(NORM_THREADS), 0, c->stream,
                        c->partial, nparts, c->scal, mb_dev(c), c->gate_ptr,
@@ -1059,6 +1060,14 @@ static int launch_batch(qdeq_ctx *c, int k) { return launch_batch_of(c, c->mg, k
 // sample).
 static int launch_batch_of(qdeq_ctx *c, MgSolve &m, int k, bool allow_sample) {
   m.sample = false;
+  // the stop test of every V-cycle but the batch's last (and one that could
+  // reach the V-cycle cap) runs in the next V-cycle's level-0 down kernel
+  const bool fold = fold_fits(c);
+  bool folded = false;
+  struct FoldReset {  // also on an error return: no later launch folds
+    qdeq_ctx *c;
+    ~FoldReset() { c->fold_next = c->skip_gate = false; }
+  } fold_reset{c};
   for (int i = 0; i < k; i++) {
     // kernel timing samples every kTimingEvery-th V-cycle: that one is
     // launched directly (same kernels and arguments as the graph) with the
@@ -1066,6 +1075,10 @@ static int launch_batch_of(qdeq_ctx *c, MgSolve &m, int k, bool allow_sample) {
     const bool sample = allow_sample && c->timing != 0 && c->timing != QDEQ_TIME_FAULT &&
                         use_tiled(c) && (c->timed_seq++ % kTimingEvery) == 0;
     m.launched++;
+    c->fold_next = folded;
+    c->skip_gate = fold && i + 1 < k && m.launched < c->cfg.max_vcycles;
+    folded = c->skip_gate;
+    if (folded) c->folded_tests++;
     if (sample) {
       c->emit_events = true;
       c->gate_ptr = c->gate;
@@ -1083,6 +1096,7 @@ static int launch_batch_of(qdeq_ctx *c, MgSolve &m, int k, bool allow_sample) {
     } else {
       QDEQ_HIP(c, hipGraphLaunch(c->g_cycle, c->stream));
     }
+    c->fold_next = c->skip_gate = false;
   }
   c->seq_issued += k;
   m.seq_target = c->seq_issued;

@@ -533,6 +533,9 @@ struct StartArgs {
   // gate[6]; START_TEST abandons the start when gate[6] is 0 (gate[0] = 1
   // stops the batch behind it, the mailbox step carries outcome 4)
   int spec;
+  // DOWN_PLAIN at level 0: the stop test of the previous V-cycle (that of
+  // k_norm_gate, on its nb <= 256 norm partials at part) runs here
+  int fold;
 };
 
 // Pre-smoothing + residual + restriction (levels above the tail).
@@ -542,9 +545,14 @@ __global__ __launch_bounds__(TILE_THREADS) void k_down(
     const T *__restrict__ cx, const T *__restrict__ cz, int N, int K, int P,
     T DH2, T relax, int nu, int zero_u, T *__restrict__ r2, int Nc, int Kc,
     int Pc, int TX, int TY, const int *__restrict__ gate, StartArgs sa) {
-  if (SMODE == DOWN_PLAIN && gate && gate[0]) return;  // converged: skip
   const bool wg0 = blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0 &&
                    threadIdx.y == 0;
+  const bool folding = SMODE == DOWN_PLAIN && !HAS_F && sa.fold;
+  if (SMODE == DOWN_PLAIN && gate && gate[0]) {  // converged: skip
+    // a folded stop test still takes its step of the mailbox sequence
+    if (folding && wg0) sa.gate[3] = sa.gate[3] + 1;
+    return;
+  }
   if (SMODE == START_PARTIALS && sa.spec) {
     const int ok = sa.gate[2] == 1 ? 1 : 0;
     if (wg0) sa.gate[6] = ok;
@@ -605,9 +613,45 @@ __global__ __launch_bounds__(TILE_THREADS) void k_down(
   }
   const int H = 2 * nu + 2;
   Tile<T> t = make_tile<T>(smem, TX, TY, H, HAS_F);
+  // folded stop test: each of k_norm_gate's 256 threads (the first four
+  // wavefronts here) loads its partial before the tile load and sums after
+  // it, in k_norm_gate's order, so every workgroup takes the same decision
+  const int ltid = threadIdx.y * blockDim.x + threadIdx.x;
+  double fold_p = 0.0;
+  if (folding && ltid < sa.nb) fold_p = sa.part[ltid];
   tile_load<T, HAS_F>(t, uin, f, cx, cz, N, K, P, zero_u != 0);
+  if (folding && ltid < 256) {
+    const double w = wave_sum(0.0 + fold_p);
+    if ((ltid & 63) == 0) sdec[ltid >> 6] = w;
+  }
   __syncthreads();
   if (SMODE == START_TEST && sdec[0] != 0.0) return;  // no V-cycle needed
+  if (folding) {
+    const double v = sqrt(((sdec[0] + sdec[1]) + sdec[2]) + sdec[3]);
+    const double thr = sa.rel ? sa.tol * sa.scal[SC_NORM0] : sa.tol;
+    int code = 0;
+    if (isnan(v) || isinf(v))
+      code = 2;
+    else if (!(v > thr))
+      code = 1;
+    if (wg0) {
+      // what k_norm_gate records (the V-cycle cap is left to it: the host
+      // folds no test that could reach the cap)
+      const int cnt = sa.gate[1] + 1;
+      sa.gate[1] = cnt;
+      sa.gate[2] = code;
+      sa.gate[3] = sa.gate[3] + 1;
+      sa.scal[SC_NORM] = v;
+      if (code) {
+        sa.gate[0] = 1;
+        sa.mb[SC_NORM] = v;
+        sa.mb[MB_COUNT] = (double)cnt;
+        sa.mb[MB_CODE] = (double)code;
+        __threadfence_system();
+      }
+    }
+    if (code) return;
+  }
   if (SMODE == START_PARTIALS) {
     // the start as loaded: residual norm and boundary-data norm partials
     double v2[2] = {0.0, 0.0};
@@ -1168,6 +1212,18 @@ static void launch_down(qdeq_ctx *c, LevelBufs<T> *B, int l, bool has_f,
     sa.cap = c->cfg.max_vcycles;
     sa.tol = c->cfg.tol;
   }
+  if (smode == DOWN_PLAIN && l == 0 && !has_f && c->fold_next && c->gate_ptr) {
+    const TileShape u0 = tile_shape(c, 0, false);
+    sa.fold = 1;
+    sa.part = c->partial;
+    sa.nb = ((d.K + u0.tx - 1) / u0.tx) * ((d.N + u0.ty - 1) / u0.ty);
+    sa.scal = c->scal;
+    sa.mb = mb_dev(c);
+    sa.gate = c->gate;
+    sa.rel = c->cfg.tol_mode == QDEQ_TOL_REL ? 1 : 0;
+    sa.cap = c->cfg.max_vcycles;
+    sa.tol = c->cfg.tol;
+  }
 #define DOWN_ARGS                                                                   \
   grd, blk, lds, c->stream, B[l].u, B[l].r, B[l].f, B[l].cx, B[l].cz, d.N, d.K, d.P, \
       DH2, relax, nu, zero_u ? 1 : 0, B[l + 1].f, dc.N, dc.K, dc.P, TX, TY,          \
@@ -1181,6 +1237,20 @@ static void launch_down(qdeq_ctx *c, LevelBufs<T> *B, int l, bool has_f,
   else
     hipLaunchKernelGGL((k_down<T, false, EXACT, DOWN_PLAIN>), DOWN_ARGS);
 #undef DOWN_ARGS
+}
+
+// Whether a batch launched kernel by kernel may fold its stop tests into the
+// next V-cycle's level-0 down kernel: the FP64 tiled cycle, at most 256
+// level-0 up tiles (k_norm_gate's threads each sum one partial) and a
+// level-0 down workgroup of four wavefronts or more.
+bool fold_fits(const qdeq_ctx *c) {
+  if (c->fold_off || c->cfg.precision != QDEQ_PREC_FP64 || !use_tiled(c) ||
+      !launch_direct(c) || tail_entry_level(c) < 1)
+    return false;
+  const LevelDims &d = c->lv[0];
+  const TileShape dn = tile_shape(c, 0, true), up = tile_shape(c, 0, false);
+  const long nb = (long)((d.K + up.tx - 1) / up.tx) * ((d.N + up.ty - 1) / up.ty);
+  return nb <= 256 && TX_THREADS * dn.tyt >= 256;
 }
 
 // Whether the start of the next FP64 solve may be folded into its first

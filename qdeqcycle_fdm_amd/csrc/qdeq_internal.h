@@ -153,6 +153,13 @@ struct qdeq_ctx {
   long long spec_taken = 0, spec_abandoned = 0;
   long long spec_taken_next = 0;  // of them, stage 2 of the next step
   bool stage_graphs_off = false;  // QDEQ_STAGE_GRAPHS=0 (see stage_graphs)
+  // Stop tests folded into the next V-cycle (see fold_fits): while a batch
+  // is launched kernel by kernel, skip_gate drops the stop-test kernel after
+  // a V-cycle that is not the batch's last, and fold_next makes the next
+  // V-cycle's level-0 down kernel run that test
+  bool skip_gate = false, fold_next = false;
+  bool fold_off = false;        // QDEQ_FOLD_STOP=0
+  long long folded_tests = 0;   // stop tests run in a next V-cycle
   int last_it = 2;             // V-cycles of the last solve (first batch size)
   // RK43 solves: the stage's key (stage s, +8 for the solve after an
   // accepted step) while its solve runs, else -1, and the V-cycles of the
@@ -297,6 +304,7 @@ void vcycle_tiled_fp32_ir(qdeq_ctx *c);
 int tiled_setup(qdeq_ctx *c);
 bool tiles_fit(const qdeq_ctx *c, int tx, int ty);
 bool fused_start_fits(const qdeq_ctx *c);
+bool fold_fits(const qdeq_ctx *c);
 int autotune_tiles(qdeq_ctx *c, int reps);
 int set_tile_shapes(qdeq_ctx *c, int nlevels, const int *shapes);
 int kernel_breakdown(qdeq_ctx *c, int reps, int *n, double *us);

@@ -45,6 +45,7 @@ EXPORTS = (
     "qdeq_output_open", "qdeq_output_close", "qdeq_events_enable",
     "qdeq_events_get", "qdeq_projection_enable", "qdeq_projection_stats",
     "qdeq_projection_fit", "qdeq_projection_inject", "qdeq_launch_ahead_stats",
+    "qdeq_folded_stop_tests",
     # include/qdeq_guess.h
     "qdeq_guess_default_config", "qdeq_train_default_config",
     "qdeq_guess_create", "qdeq_guess_destroy", "qdeq_guess_last_error",
@@ -232,6 +233,7 @@ def load(path: str = LIB_PATH):
     L.qdeq_projection_enable.argtypes = [_vp, ctypes.c_int]
     L.qdeq_projection_stats.argtypes = [_vp, _llp, _llp, _dp, _ip]
     L.qdeq_launch_ahead_stats.argtypes = [_vp, _llp, _llp, _llp]
+    L.qdeq_folded_stop_tests.argtypes = [_vp, _llp]
     L.qdeq_projection_fit.argtypes = [_vp, _dp, _dp, ctypes.c_int, _dp]
     L.qdeq_projection_inject.argtypes = [_vp, ctypes.c_double]
     for name in EXPORTS:
@@ -754,7 +756,15 @@ def _engine_launch_ahead_stats(self) -> dict:
     return dict(taken=t.value, abandoned=a.value, next_step=n.value)
 
 
+def _engine_folded_stop_tests(self) -> int:
+    """Stop tests run inside the next V-cycle of their batch so far."""
+    n = ctypes.c_longlong(0)
+    self._check(load().qdeq_folded_stop_tests(self.h, ctypes.byref(n)))
+    return n.value
+
+
 Engine.launch_ahead_stats = _engine_launch_ahead_stats
+Engine.folded_stop_tests = _engine_folded_stop_tests
 Engine.projection_fit = _engine_projection_fit
 Engine.projection_inject = lambda self, factor: self._check(
     load().qdeq_projection_inject(self.h, factor))

@@ -479,3 +479,70 @@ def test_next_step_launched_ahead_ends_at_tmax_and_stays_off_between_records(
             assert sa["next_step"] > 10
         else:
             assert sa["next_step"] == 0 and sa["taken"] > 0
+
+
+SHAPES_1025 = [((130, 34, 16), (66, 66, 16)), ((34, 34, 16), (34, 34, 16)),
+               ((32, 10, 8), (34, 10, 8)), ((12, 12, 8), (12, 8, 8)),
+               ((10, 6, 8), (18, 2, 8))]
+
+
+@pytest.mark.parametrize("n,levels,steps", [(1025, 7, 8), (257, 5, 60)])
+def test_stop_tests_folded_equal_separate_runs(n, levels, steps, monkeypatch):
+    """Within a batch launched kernel by kernel, the stop test of every
+    V-cycle but the batch's last runs in the next V-cycle's level-0 down
+    kernel (every workgroup sums the norm partials in the stop-test kernel's
+    order).  Bit for bit the runs of QDEQ_FOLD_STOP=0, which launch a
+    stop-test kernel after every V-cycle, V-cycle counts included; tests were
+    folded (at 1025^2 with the 66x66 level-0 tiles: 256 partials)."""
+    pb = config.make_bp1_problem(n, basin=True, levels=levels,
+                                 tol_mode=config.TOL_REL, tol=1e-10)
+    me = lib.build_mu_equi(pb.mu)
+    runs = []
+    for fold in (True, False):
+        if fold:
+            monkeypatch.delenv("QDEQ_FOLD_STOP", raising=False)
+        else:
+            monkeypatch.setenv("QDEQ_FOLD_STOP", "0")
+        with lib.Engine(pb, me, **PROD) as eng:
+            eng.set_tiles(True, 64, 16)
+            if n == 1025:
+                eng.set_tile_shapes(SHAPES_1025)
+            eng.rk43_setup()
+            eng.rk43_step(steps)
+            runs.append((eng.state(), eng.folded_stop_tests()))
+    (a, fa), (b, fb) = runs
+    for key in ("t", "dt", "vcycles", "solves", "attempts"):
+        assert a["info"][key] == b["info"][key], key
+    for k in ("slip", "psi", "V", "U"):
+        np.testing.assert_array_equal(a[k], b[k])
+    assert fb == 0
+    # batches of two or more V-cycles (few in the early steps, where most
+    # solves need one)
+    assert fa > 0
+
+
+@pytest.mark.parametrize("cap", [2, 3, 5])
+def test_folded_stop_tests_stop_at_the_cap(cap, monkeypatch):
+    """A solve that cannot meet its tolerance stops at the V-cycle cap with
+    the same result whether its stop tests are folded or not: no test that
+    could reach the cap is folded."""
+    pb = config.make_bp1_problem(129, basin=True, levels=4)
+    me = oracle.mu_equi(pb.mu)
+    U = _perturbed(pb, 11)
+    out = []
+    for fold in (True, False):
+        if fold:
+            monkeypatch.delenv("QDEQ_FOLD_STOP", raising=False)
+        else:
+            monkeypatch.setenv("QDEQ_FOLD_STOP", "0")
+        with lib.Engine(pb, me, **{**PROD, "tol": 1e-300, "max_vcycles": cap}) as eng:
+            eng.set_tiles(True, 64, 16)
+            V = U.copy()
+            with pytest.raises(lib.QdeqError) as ei:
+                eng.ode_solve(V)
+            assert ei.value.status == lib.QDEQ_ERR_MG_NOT_CONVERGED
+            out.append((V, eng.folded_stop_tests()))
+    (va, fa), (vb, fb) = out
+    np.testing.assert_array_equal(va, vb)
+    assert fb == 0
+    assert fa >= 1
