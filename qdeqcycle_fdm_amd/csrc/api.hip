@@ -768,11 +768,21 @@ static int rk_readback_ready(qdeq_ctx *c, bool block, int *ready) {
 // device, no candidate start, no fault-update timing), with the current
 // solve's V-cycles in flight and nothing else launched ahead.
 // QDEQ_SPECULATE=0 turns it off.
-static bool spec_allowed(const qdeq_ctx *c) {
+static bool spec_base_allowed(const qdeq_ctx *c) {
   return !c->spec_off && launch_direct(c) && !stage_graphs(c) && device_start(c) &&
          c->cfg.precision == QDEQ_PREC_FP64 && c->timing != QDEQ_TIME_FAULT &&
          c->proj_slots == 0 && (c->guess == nullptr || c->guess_mode == QDEQ_GUESS_OFF) &&
-         c->mg.phase == 2 && c->mg.dev_start && !c->mg.cand && !c->spec_pending;
+         !c->spec_pending;
+}
+static bool spec_allowed(const qdeq_ctx *c) {
+  return spec_base_allowed(c) && c->mg.phase == 2 && c->mg.dev_start && !c->mg.cand;
+}
+
+// Whether the accepted attempt's commit and final stage-1 solve may be
+// queued behind the combination, before the host has read the embedded
+// error: the same single-simulation conditions, no solve in flight.
+static bool accept_spec_allowed(const qdeq_ctx *c) {
+  return spec_base_allowed(c) && c->mg.phase == 0;
 }
 
 // The step size an accepted step hands to the next one (odeSolver.cpp:1621-1632)
@@ -937,6 +947,19 @@ static int rk_advance(qdeq_ctx *c, bool block, int *done) {
           } else {
             launch_combine(c);
             if ((st = rk_readback(c, true))) return st;
+            if (accept_spec_allowed(c)) {
+              // the accepted case queued behind the readback, gated on the
+              // combination's own verdict (gate[7]): the commit, the final
+              // stage 1's state, and that solve's start and first batch
+              launch_commit(c, c->t + c->dt, c->gate);
+              launch_stage(c, 1, c->gate);
+              const int key = c->solve_key;
+              c->solve_key = 8 + 1;
+              st = mg_begin_spec(c, 7);
+              c->solve_key = key;
+              if (st) return st;
+              r.spec = true;
+            }
             r.phase = RK_WAIT_COMBINE;
           }
         } else if (!c->rk_ready) {
@@ -960,6 +983,13 @@ static int rk_advance(qdeq_ctx *c, bool block, int *done) {
         if ((st = check_flags(c))) return st;
         c->totErr = c->scal_host[SC_TOTERR];
         bool accept = c->totErr < p.totTol;
+        if (r.spec && !accept) {
+          // rejected on the device too: the work queued for the accepted
+          // case did nothing (its start abandoned itself)
+          r.spec = false;
+          c->spec_pending = false;
+          c->spec_abandoned++;
+        }
         if (!accept) {
           c->dt = qdeq_compute_step_size(c->it, c->dt, p.mindt, p.maxdt, c->totErr,
                                          p.totTol, c->errA);
@@ -971,7 +1001,8 @@ static int rk_advance(qdeq_ctx *c, bool block, int *done) {
         }
         c->attempts = r.attempts;
         c->t = c->t + c->dt;  // odeSolver.cpp:1605-1619
-        launch_commit(c, c->t);
+        // already queued (with the final stage 1) when the device accepted
+        if (!r.spec) launch_commit(c, c->t);
         r.s = 1;
         r.final_stage = true;
         r.phase = RK_STAGE_BEGIN;

@@ -193,11 +193,14 @@ __device__ inline void stage_node(int k, int s, double dt, double t, double rati
 }
 
 // tdt: t and dt of the attempt on the device
+// go: launched ahead of the accept decision, it runs only if the attempt was
+// accepted on the device (go[7] == 1, written by k_combine)
 __global__ void k_stage(int s, const double *tdt, double ratio, double Vini,
                         const double *slip, const double *psi,
                         const double *ss, double *const *FV,
                         double *const *Fd, double *Ys, double *Yp, double *u,
-                        int nk, int K, int P) {
+                        int nk, int K, int P, const int *go) {
+  if (go && go[7] != 1) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
   if (k > nk) return;
   stage_node(k, s, tdt[1], tdt[0], ratio, Vini, slip, psi, ss, FV, Fd, Ys, Yp, u, K,
@@ -325,7 +328,7 @@ __global__ void k_zero_rates(double *const *FV, double *const *Fd, int nz,
 __global__ __launch_bounds__(1024) void k_combine(
     const double *tdt, const double *slip, const double *psi, double *const *FV,
     double *const *Fd, double *y3s, double *y3p, double *y4s, double *y4p,
-    int nk, double *out) {
+    int nk, double *out, int *gate, double totTol) {
   __shared__ double s1[1024], s2[1024];
   const int tid = threadIdx.x;
   const double dt = tdt[1];
@@ -373,7 +376,11 @@ __global__ __launch_bounds__(1024) void k_combine(
   }
   if (tid == 0) {
     const double r1 = sqrt(e1), r2 = sqrt(e2);
-    out[0] = r1 / (sqrt((double)nk) * 1.0) + r2 / (sqrt((double)nk) * 1.0);
+    const double err = r1 / (sqrt((double)nk) * 1.0) + r2 / (sqrt((double)nk) * 1.0);
+    out[0] = err;
+    // the accept test the host applies first (odeSolver.cpp:1596), for
+    // work launched ahead of the host's decision
+    gate[7] = err < totTol ? 1 : 0;
   }
 }
 
@@ -381,7 +388,8 @@ __global__ __launch_bounds__(1024) void k_combine(
 // becomes t_new = t + dt.
 __global__ void k_commit(double *slip, double *psi, const double *y4s,
                          const double *y4p, double *V1, double *d1, int nk,
-                         double *tdt, double t_new) {
+                         double *tdt, double t_new, const int *go) {
+  if (go && go[7] != 1) return;  // launched ahead: only if accepted
   const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
   if (k == 1) tdt[0] = t_new;
   if (k > nk) return;
@@ -479,12 +487,12 @@ static double *const *dev_FV(qdeq_ctx *c) {
 }
 static double *const *dev_Fd(qdeq_ctx *c) { return dev_FV(c) + 7; }
 
-void launch_stage(qdeq_ctx *c, int s) {
+void launch_stage(qdeq_ctx *c, int s, const int *go) {
   FaultBufs &f = c->fb;
   hipLaunchKernelGGL(k_stage, nodes_grid(c->cfg.nk), dim3(FT), 0, c->stream, s,
                      c->scal + SC_T, kC[s], c->cfg.Vini, f.slip, f.psi, f.slip_shift,
                      dev_FV(c), dev_Fd(c), f.Ys[s], f.Yp[s], c->d[0].u,
-                     c->cfg.nk, c->lv[0].K, c->lv[0].P);
+                     c->cfg.nk, c->lv[0].K, c->lv[0].P, go);
 }
 
 void launch_fault_update(qdeq_ctx *c, int s) {
@@ -530,14 +538,14 @@ void launch_combine(qdeq_ctx *c) {
   FaultBufs &f = c->fb;
   hipLaunchKernelGGL(k_combine, dim3(1), dim3(1024), 0, c->stream, c->scal + SC_T, f.slip,
                      f.psi, dev_FV(c), dev_Fd(c), f.y3s, f.y3p, f.y4s, f.y4p,
-                     c->cfg.nk, c->scal + SC_TOTERR);
+                     c->cfg.nk, c->scal + SC_TOTERR, c->gate, c->cfg.totTol);
 }
 
-void launch_commit(qdeq_ctx *c, double t_new) {
+void launch_commit(qdeq_ctx *c, double t_new, const int *go) {
   FaultBufs &f = c->fb;
   hipLaunchKernelGGL(k_commit, nodes_grid(c->cfg.nk), dim3(FT), 0, c->stream,
                      f.slip, f.psi, f.y4s, f.y4p, f.FV[1], f.Fd[1], c->cfg.nk,
-                     c->scal + SC_T, t_new);
+                     c->scal + SC_T, t_new, go);
 }
 
 void launch_maxima(qdeq_ctx *c) {

@@ -305,9 +305,10 @@ __global__ __launch_bounds__(NORM_THREADS) void k_start_partials(
     const double *__restrict__ ui, const double *__restrict__ u,
     const double *__restrict__ cand, const double *__restrict__ cx,
     const double *__restrict__ cz, int N, int K, int P, double DH2, int bd,
-    double *__restrict__ partial, const int *__restrict__ spec) {
-  // a start launched ahead: only if the solve before it converged
-  if (spec && spec[2] != 1) return;
+    double *__restrict__ partial, const int *__restrict__ spec, int go) {
+  // a start launched ahead: only if what it waits on went through (spec[go]:
+  // the solve before it converged, or its attempt was accepted)
+  if (spec && spec[go] != 1) return;
   __shared__ double sh[3][NORM_THREADS];
   double acc = 0.0, accc = 0.0, accb = 0.0;
   // each thread's points of a row are evaluated together (their loads in
@@ -373,7 +374,8 @@ __global__ __launch_bounds__(NORM_THREADS) void k_start_partials(
 __global__ __launch_bounds__(NORM_THREADS) void k_start_gate(
     const double *__restrict__ partial, int nb, int cand, int rel, double tol, int cap,
     double *__restrict__ scal, double *__restrict__ mb, int *__restrict__ gate, int spec) {
-  if (spec && gate[2] != 1) {
+  // spec: 0, or the gate word a start launched ahead waits on (2 or 7)
+  if (spec && gate[spec] != 1) {
     if (threadIdx.x == 0) {
       gate[0] = 1;
       const int seq = gate[3] + 1;
@@ -815,11 +817,12 @@ static void launch_plain_dev_start(qdeq_ctx *c, bool fuse) {
     hipLaunchKernelGGL(k_start_partials<false>, dim3(norm_blocks(c)), dim3(NORM_THREADS),
                        0, c->stream, c->d[0].u, c->d[0].u, (const double *)nullptr,
                        c->d[0].cx, c->d[0].cz, d.N, d.K, d.P, d.DH * d.DH, rel,
-                       c->partial, (const int *)(c->spec_launch ? c->gate : nullptr));
+                       c->partial, (const int *)(c->spec_launch ? c->gate : nullptr),
+                       c->spec_go);
   }
   hipLaunchKernelGGL(k_start_gate, dim3(1), dim3(NORM_THREADS), 0, c->stream,
                      c->partial, norm_blocks(c), 0, rel, c->cfg.tol, c->cfg.max_vcycles,
-                     c->scal, mb_dev(c), c->gate, c->spec_launch ? 1 : 0);
+                     c->scal, mb_dev(c), c->gate, c->spec_launch ? c->spec_go : 0);
 }
 
 // V-cycles of the first batch of a solve: what the same RK43 stage needed in
@@ -879,7 +882,8 @@ int mg_begin(qdeq_ctx *c) {
     } else {
       hipLaunchKernelGGL(k_start_partials<true>, dim3(norm_blocks(c)), dim3(NORM_THREADS), 0,
                          c->stream, plain, c->d[0].u, cand, c->d[0].cx, c->d[0].cz,
-                         d.N, d.K, d.P, d.DH * d.DH, rel, c->partial, (const int *)nullptr);
+                         d.N, d.K, d.P, d.DH * d.DH, rel, c->partial, (const int *)nullptr,
+                         2);
       hipLaunchKernelGGL(k_start_gate, dim3(1), dim3(NORM_THREADS), 0, c->stream,
                          c->partial, norm_blocks(c), 1, rel, c->cfg.tol,
                          c->cfg.max_vcycles, c->scal, mb_dev(c), c->gate, 0);
@@ -982,17 +986,19 @@ int mg_begin_stage(qdeq_ctx *c, int kind, int s) {
 // start abandons itself (outcome 4) and mg_poll resumes the gate before the
 // continuation.  The same kernels and arguments as mg_begin's, so a solve
 // launched ahead and taken over computes what it would have computed.
-int mg_begin_spec(qdeq_ctx *c) {
+int mg_begin_spec(qdeq_ctx *c, int go) {
   MgSolve &m = c->mg_next;
   m = MgSolve{};
   m.dev_start = true;
   m.page = 1 - c->mg.page;
   c->mb_page = m.page;
   c->spec_launch = true;
+  c->spec_go = go;
   launch_plain_dev_start(c, fused_start_fits(c));
   c->seq_issued += 1;
   const int st = launch_batch_of(c, m, first_batch(c), true);
   c->spec_launch = false;
+  c->spec_go = 2;
   c->fuse_start_next = false;
   c->mb_page = c->mg.page;
   if (st != QDEQ_OK) return st;

@@ -528,10 +528,11 @@ struct StartArgs {
   int *gate;
   int rel, cap;
   double tol;
-  // launched ahead, behind a solve still in flight: START_PARTIALS goes on
-  // only if that solve converged (gate[2] == 1) and records the verdict in
-  // gate[6]; START_TEST abandons the start when gate[6] is 0 (gate[0] = 1
-  // stops the batch behind it, the mailbox step carries outcome 4)
+  // launched ahead (spec: the gate word it waits on, else 0): START_PARTIALS
+  // goes on only if gate[spec] == 1 (2: the solve before it converged; 7:
+  // k_combine accepted the attempt) and records the verdict in gate[6];
+  // START_TEST abandons the start when gate[6] is 0 (gate[0] = 1 stops the
+  // batch behind it, the mailbox step carries outcome 4)
   int spec;
   // DOWN_PLAIN at level 0: the stop test of the previous V-cycle (that of
   // k_norm_gate, on its nb <= 256 norm partials at part) runs here
@@ -554,7 +555,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_down(
     return;
   }
   if (SMODE == START_PARTIALS && sa.spec) {
-    const int ok = sa.gate[2] == 1 ? 1 : 0;
+    const int ok = sa.gate[sa.spec] == 1 ? 1 : 0;
     if (wg0) sa.gate[6] = ok;
     if (!ok) return;
   }
@@ -1207,7 +1208,7 @@ static void launch_down(qdeq_ctx *c, LevelBufs<T> *B, int l, bool has_f,
     sa.scal = c->scal;
     sa.mb = mb_dev(c);
     sa.gate = c->gate;
-    sa.spec = c->spec_launch ? 1 : 0;
+    sa.spec = c->spec_launch ? c->spec_go : 0;
     sa.rel = c->cfg.tol_mode == QDEQ_TOL_REL ? 1 : 0;
     sa.cap = c->cfg.max_vcycles;
     sa.tol = c->cfg.tol;

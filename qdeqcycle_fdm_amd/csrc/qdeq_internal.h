@@ -149,6 +149,9 @@ struct qdeq_ctx {
   bool spec_pending = false;
   int mb_page = 0;
   bool spec_launch = false;
+  // the gate word a solve launched ahead checks before it starts: 2 (the
+  // solve before it converged) or 7 (k_combine accepted the attempt)
+  int spec_go = 2;
   bool spec_off = false;       // QDEQ_SPECULATE=0
   long long spec_taken = 0, spec_abandoned = 0;
   long long spec_taken_next = 0;  // of them, stage 2 of the next step
@@ -263,7 +266,7 @@ int mg_begin(qdeq_ctx *c);
 // kind 1: stage s - 1's fault update and stage s's state)
 bool stage_graphs(const qdeq_ctx *c);
 int mg_begin_stage(qdeq_ctx *c, int kind, int s);
-int mg_begin_spec(qdeq_ctx *c);
+int mg_begin_spec(qdeq_ctx *c, int go = 2);
 int mg_poll(qdeq_ctx *c, bool block, int *done);
 int mg_build_coefficients(qdeq_ctx *c);
 void mg_destroy_graphs(qdeq_ctx *c);
@@ -461,7 +464,7 @@ void launch_nr(qdeq_ctx *c, const double *Ts, const double *Tn,
 void launch_dpsi(qdeq_ctx *c, const double *psi, const double *V,
                  const double *b, double *dpsi);
 // stage kernels: t and dt from scal[SC_T], scal[SC_DT]
-void launch_stage(qdeq_ctx *c, int s);
+void launch_stage(qdeq_ctx *c, int s, const int *go = nullptr);
 // fault update of stage s and the stage state of s + 1 in one launch
 void launch_update_stage(qdeq_ctx *c, int s, const int *spec = nullptr);
 void launch_fault_update(qdeq_ctx *c, int s);
@@ -470,7 +473,7 @@ void launch_zero_stage_rates(qdeq_ctx *c, double t, double dt);
 void launch_update_next(qdeq_ctx *c, double t, double dt, const int *spec);
 void launch_combine(qdeq_ctx *c);
 // accepts y4 and sets the device time to t_new
-void launch_commit(qdeq_ctx *c, double t_new);
+void launch_commit(qdeq_ctx *c, double t_new, const int *go = nullptr);
 void launch_maxima(qdeq_ctx *c);
 
 }  // namespace qdeq

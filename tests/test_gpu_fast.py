@@ -546,3 +546,36 @@ def test_folded_stop_tests_stop_at_the_cap(cap, monkeypatch):
     np.testing.assert_array_equal(va, vb)
     assert fb == 0
     assert fa >= 1
+
+
+def test_accepted_case_launched_ahead_equal_waiting_runs_with_rejections(monkeypatch):
+    """Behind each attempt's combination the commit, the final stage 1 and
+    that solve's start and first batch are queued, gated on the device's own
+    accept test; a rejected attempt finds them skipped.  Started with a first
+    step far too long for the error tolerance (a series of rejected attempts
+    until the controller has shrunk it), the run is bit for bit that of
+    QDEQ_SPECULATE=0, V-cycle counts included."""
+    pb = config.make_bp1_problem(129, basin=True, levels=4,
+                                 tol_mode=config.TOL_REL, tol=1e-10)
+    me = oracle.mu_equi(pb.mu)
+    steps = 80
+    runs = []
+    for spec in (True, False):
+        if spec:
+            monkeypatch.delenv("QDEQ_SPECULATE", raising=False)
+        else:
+            monkeypatch.setenv("QDEQ_SPECULATE", "0")
+        with lib.Engine(pb, me, **{**PROD, "ini_dt": 1e7}) as eng:
+            eng.set_tiles(True, 64, 16)
+            eng.rk43_setup()
+            eng.rk43_step(steps)
+            runs.append((eng.state(), eng.launch_ahead_stats()))
+    (a, sa), (b, sb) = runs
+    for key in ("t", "dt", "vcycles", "solves", "attempts"):
+        assert a["info"][key] == b["info"][key], key
+    for k in ("slip", "psi", "V", "U"):
+        np.testing.assert_array_equal(a[k], b[k])
+    attempts = (a["info"]["solves"] - steps - 1) / 5
+    assert attempts >= steps + 3         # rejected attempts happened
+    assert sb["taken"] == 0
+    assert sa["abandoned"] > 0 and sa["taken"] > 0
