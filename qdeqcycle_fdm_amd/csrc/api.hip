@@ -763,17 +763,18 @@ static int rk_readback_ready(qdeq_ctx *c, bool block, int *ready) {
   return QDEQ_OK;
 }
 
-// Whether the next stage may be launched ahead of the current solve's end:
-// a single simulation launching kernel by kernel (FP64, start decided on the
-// device, no candidate start, no fault-update timing), with the current
-// solve's V-cycles in flight and nothing else launched ahead.
-// QDEQ_SPECULATE=0 turns it off.
+// Work may be launched ahead of a decision the host has not read yet for a
+// single simulation launching kernel by kernel (FP64, start decided on the
+// device, no candidate start, no fault-update timing) with nothing else
+// launched ahead.  QDEQ_SPECULATE=0 turns it off.
 static bool spec_base_allowed(const qdeq_ctx *c) {
   return !c->spec_off && launch_direct(c) && !stage_graphs(c) && device_start(c) &&
          c->cfg.precision == QDEQ_PREC_FP64 && c->timing != QDEQ_TIME_FAULT &&
          c->proj_slots == 0 && (c->guess == nullptr || c->guess_mode == QDEQ_GUESS_OFF) &&
          !c->spec_pending;
 }
+// The next stage behind the current solve: its V-cycles in flight, started
+// on the device from the plain start.
 static bool spec_allowed(const qdeq_ctx *c) {
   return spec_base_allowed(c) && c->mg.phase == 2 && c->mg.dev_start && !c->mg.cand;
 }
