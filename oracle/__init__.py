@@ -196,6 +196,12 @@ def residual(r, u, f, me, ni, nk, dh, level):
     lib().orc_residual(_p(r), _p(_c(u)), _p(_c(f)), _p(_c(me)), ni, nk, dh, level)
 
 
+def nrm2(x) -> float:
+    """Euclidean norm as the oracle's solves take it (orc_nrm2)."""
+    x = _c(x)
+    return lib().orc_nrm2(_p(x), x.size)
+
+
 def restrict(r2, r, e2, ni, nk, level):
     lib().orc_restrict(_p(r2), _p(_c(r)), _p(e2), ni, nk, level)
 

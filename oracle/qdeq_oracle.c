@@ -420,7 +420,9 @@ void orc_prolong(double *u, const double *e2, int ni, int nk, int level) {
 
 /* Euclidean norm standing in for cublasDnrm2 (odeSolver.cpp:694,700); the
  * reference's library routine has no published summation order, so only
- * the stop test depends on it. */
+ * the stop test depends on it.  Pinned by closed forms, exact scaling by
+ * powers of two and the rounding bound of a sequential sum of n positive
+ * terms (tests/test_oracle.py). */
 double orc_nrm2(const double *x, long n) {
   double s = 0.0;
   for (long i = 0; i < n; i++) s += x[i] * x[i];

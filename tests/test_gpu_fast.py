@@ -82,6 +82,12 @@ def test_fast_fp64_relative_tolerance_solve():
     assert r <= 1e-10 * r0
     np.testing.assert_array_equal(got, ref)
     np.testing.assert_allclose(r0, r0_ref, rtol=1e-13)
+    # the final norm from the level-0 up kernel's tile partials (fixed
+    # wavefront and tile order) against the oracle's sequential sum over the
+    # same, bit-identical residual: both lie within (n + 64) u of the exact
+    # norm of n points (u = 2^-53)
+    n_pts = pb.params.ni * pb.params.nk
+    np.testing.assert_allclose(r, r_ref, rtol=2 * (n_pts + 64) * 2.0 ** -53)
 
 
 @pytest.mark.parametrize("tiled", [False, True])

@@ -371,6 +371,39 @@ def test_vcycle_contracts_residual():
     assert ratios[0] < 1e-3, ratios
 
 
+def test_nrm2_closed_forms_scaling_and_rounding_bound():
+    """orc_nrm2, the norm of every stop test in the oracle's solves (standing
+    in for cublasDnrm2, odeSolver.cpp:694,700, whose summation order is not
+    published): Pythagorean closed forms and unit vectors (a dropped square
+    root or square, an absolute-value sum, a skipped first or last element
+    each fail one), exact scaling by powers of two, and the norm of the
+    exactly summed rounded squares within the error bound of a sequential sum
+    of n positive terms, u (n + 4) relative with u = 2^-53."""
+    assert oracle.nrm2(np.array([3.0, 4.0])) == 5.0
+    assert oracle.nrm2(np.array([2.0, 3.0, 6.0])) == 7.0
+    assert oracle.nrm2(np.array([1.0, -2.0, 2.0])) == 3.0
+    assert oracle.nrm2(np.array([-2.5])) == 2.5
+    assert oracle.nrm2(np.zeros(17)) == 0.0
+    for i in (0, 5, 16):
+        e = np.zeros(17)
+        e[i] = -1.0
+        assert oracle.nrm2(e) == 1.0
+    rng = np.random.default_rng(11)
+    u = 2.0 ** -53
+    for n in (1, 2, 7, 1000, 257 * 257):
+        x = rng.normal(size=n) * 10.0 ** rng.uniform(-3, 3, size=n)
+        got = oracle.nrm2(x)
+        exact = math.sqrt(math.fsum(v * v for v in x.tolist()))
+        assert abs(got - exact) <= (n + 4) * u * exact, (n, got, exact)
+        assert abs(got - np.linalg.norm(x)) <= (n + 4) * u * exact
+        for k in (-40, 7, 300):
+            assert oracle.nrm2(x * 2.0 ** k) == got * 2.0 ** k
+    # a 2-D level array is taken whole, row after row
+    a = rng.normal(size=(33, 31))
+    assert oracle.nrm2(a) == oracle.nrm2(a.ravel())
+    assert abs(oracle.nrm2(a) - np.linalg.norm(a)) <= (a.size + 4) * u * np.linalg.norm(a)
+
+
 # ------------------------------------------------------- NR and state rate
 
 def test_nr_without_damping_is_closed_form():
