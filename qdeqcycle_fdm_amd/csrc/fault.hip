@@ -40,6 +40,10 @@ constexpr double HB1 = 4586570599. / 29645900160., HB3 = 178811875. / 945068544.
 __device__ inline bool nr_node(double Ts, double Tn, double psi, double a,
                                double eta, double V0, double Vprev,
                                double *Vout) {
+  // e^(-psi/a) and 2 eta V0 e^(-psi/a) of every iteration's two expressions,
+  // formed once with their left-to-right products, so the same bits
+  const double em = qd_exp(-psi / a);
+  const double c = 2.0 * eta * V0 * em;
   for (int attempt = 0; attempt < 2; attempt++) {  // the reference retries once
     int iter = 0;
     double var_old = 0.5 * Vprev / V0 * qd_exp(psi / a);
@@ -49,9 +53,10 @@ __device__ inline bool nr_node(double Ts, double Tn, double psi, double a,
     double var_new = var_old;
     const int iter_max = 100000;
     while (iter < iter_max) {
-      const double fy = 2.0 * eta * V0 * qd_exp(-psi / a) * qd_sinh(var_old) +
-                        fa * var_old - fb;
-      const double df = 2.0 * eta * V0 * qd_exp(-psi / a) * qd_cosh(var_old) + fa;
+      double sh, ch;
+      qd_sinh_cosh(var_old, &sh, &ch);
+      const double fy = c * sh + fa * var_old - fb;
+      const double df = c * ch + fa;
       const double d = -fy / (df + 1e-100);
       var_new = var_old + d;
       const double err = fabs(d) / (fabs(var_old) + 1e-100);
@@ -61,7 +66,7 @@ __device__ inline bool nr_node(double Ts, double Tn, double psi, double a,
     }
     const double friction_update = a * var_new;
     if (!isnan(friction_update)) {
-      double V = 2.0 * V0 * qd_exp(-psi / a);
+      const double V = 2.0 * V0 * em;
       *Vout = V * qd_sinh(var_new);
       return true;
     }

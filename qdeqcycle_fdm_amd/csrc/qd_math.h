@@ -92,6 +92,47 @@ QD_HD double qd_cosh(double x) {
   return (0.5 * e) * e;
 }
 
+// qd_sinh(x) and qd_cosh(x) together: each branch does the operations of the
+// two functions above for its range, sharing the e^|x| (and 1/e^|x|) they
+// both form, so both results have the same bits as the separate calls.
+QD_HD void qd_sinh_cosh(double x, double *s, double *c) {
+  if (x != x) {
+    *s = x;
+    *c = x;
+    return;
+  }
+  const double ax = fabs(x);
+  double r, ch;
+  if (ax < 22.0) {
+    const double e = qd_exp(ax);
+    const double ie = 1.0 / e;
+    ch = 0.5 * (e + ie);
+    if (ax < 0.5) {
+      const double x2 = ax * ax;
+      double p = 1.0 / 355687428096000.0;
+      p = p * x2 + 1.0 / 1307674368000.0;
+      p = p * x2 + 1.0 / 6227020800.0;
+      p = p * x2 + 1.0 / 39916800.0;
+      p = p * x2 + 1.0 / 362880.0;
+      p = p * x2 + 1.0 / 5040.0;
+      p = p * x2 + 1.0 / 120.0;
+      p = p * x2 + 1.0 / 6.0;
+      r = ax + ax * (x2 * p);
+    } else {
+      r = 0.5 * (e - ie);
+    }
+  } else if (ax < 709.0) {
+    r = 0.5 * qd_exp(ax);
+    ch = r;
+  } else {
+    const double e = qd_exp(0.5 * ax);
+    r = (0.5 * e) * e;
+    ch = r;
+  }
+  *s = x < 0.0 ? -r : r;
+  *c = ch;
+}
+
 QD_HD double qd_log(double x) {
   if (x != x) return x;
   if (x < 0.0) return (x - x) / (x - x);
