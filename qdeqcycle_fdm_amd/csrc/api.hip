@@ -950,7 +950,7 @@ static int rk_advance(qdeq_ctx *c, bool block, int *done) {
             if ((st = rk_readback(c, true))) return st;
             if (accept_spec_allowed(c)) {
               // the accepted case queued behind the readback, gated on the
-              // combination's own verdict (gate[7]): the commit, the final
+              // combination's own accept decision (gate[7]): the commit, the final
               // stage 1's state, and that solve's start and first batch
               launch_commit(c, c->t + c->dt, c->gate);
               launch_stage(c, 1, c->gate);

@@ -530,7 +530,7 @@ struct StartArgs {
   double tol;
   // launched ahead (spec: the gate word it waits on, else 0): START_PARTIALS
   // goes on only if gate[spec] == 1 (2: the solve before it converged; 7:
-  // k_combine accepted the attempt) and records the verdict in gate[6];
+  // k_combine accepted the attempt) and records that decision in gate[6];
   // START_TEST abandons the start when gate[6] is 0 (gate[0] = 1 stops the
   // batch behind it, the mailbox step carries outcome 4)
   int spec;
