@@ -197,6 +197,32 @@ __device__ inline void stage_node(int k, int s, double dt, double t, double rati
   row[K - 1] = 0.50 * Vini * (t + ratio * dt) + ss[k];
 }
 
+// stage_node with its inputs in registers: slip, psi and SlipShift of the
+// node and the stage rates F_1 .. F_{s-1} (fv[j], fd[j]); the same
+// operations in the same order, so the same bits.
+__device__ inline void stage_node_in(int k, int s, double dt, double t, double ratio,
+                                     double Vini, double sl, double ps, double ssk,
+                                     const double (&fv)[6], const double (&fd)[6],
+                                     double *Ys, double *Yp, double *u, int K, int P) {
+  double ys = sl, yp = ps;
+  if (s >= 2) {
+    ys = sl + kA[s][1] * dt * fv[1];
+    yp = ps + kA[s][1] * dt * fd[1];
+#pragma unroll
+    for (int jj = 2; jj < 6; jj++) {
+      if (jj < s) {
+        ys = ys + kA[s][jj] * dt * fv[jj];
+        yp = yp + kA[s][jj] * dt * fd[jj];
+      }
+    }
+  }
+  Ys[k] = ys;
+  Yp[k] = yp;
+  double *row = u + (size_t)(k - 1) * P;
+  row[0] = 0.5 * ys;
+  row[K - 1] = 0.50 * Vini * (t + ratio * dt) + ssk;
+}
+
 // tdt: t and dt of the attempt on the device
 // go: launched ahead of the accept decision, it runs only if the attempt was
 // accepted on the device (go[7] == 1, written by k_combine)
@@ -215,13 +241,17 @@ __global__ void k_stage(int s, const double *tdt, double ratio, double Vini,
 // Fault update after a stage solve: Ts from the compact u (columns 0..3 are
 // the fault and its three neighbours), NR for V from the stage's previous
 // rate (zeroed, odeSolver.cpp:1484-1488), then dpsi.
-__device__ inline void update_node(int k, const double *u, int P,
-                                   const double *mu_f, const double *tau_n0,
-                                   const double *Yp, const double *a,
-                                   const double *b, const double *eta, double V0,
-                                   double f0, double L, double c0, double c1,
-                                   double c2, double c3, double *Ts, double *Tn,
-                                   double *V, double *dpsi, int *flags) {
+// Returns the node's new rate and state rate (also stored).
+struct NodeRates {
+  double V, dpsi;
+};
+__device__ inline NodeRates update_node(int k, const double *u, int P,
+                                        const double *mu_f, const double *tau_n0,
+                                        const double *Yp, const double *a,
+                                        const double *b, const double *eta, double V0,
+                                        double f0, double L, double c0, double c1,
+                                        double c2, double c3, double *Ts, double *Tn,
+                                        double *V, double *dpsi, int *flags) {
   const double *row = u + (size_t)(k - 1) * P;
   const double du = c0 * row[0] + c1 * row[1] + c2 * row[2] + c3 * row[3];
   const double ts = mu_f[k] * du * 1.0e3;
@@ -235,7 +265,9 @@ __device__ inline void update_node(int k, const double *u, int P,
     Vn = V[k];
   }
   V[k] = Vn;
-  dpsi[k] = dpsi_node(psi, Vn, b[k], f0, V0, L);
+  const double dp = dpsi_node(psi, Vn, b[k], f0, V0, L);
+  dpsi[k] = dp;
+  return NodeRates{Vn, dp};
 }
 
 __global__ void k_fault_update(const double *u, int P, int nk,
@@ -278,11 +310,34 @@ __global__ void k_update_stage(double *u, int nk, int K, int P, UpdateArgs ua,
   if (spec && spec[2] != 1) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
   if (k > nk) return;
-  update_node(k, u, P, ua.mu_f, ua.tau_n0, ua.Yp, ua.a, ua.b, ua.eta, ua.V0, ua.f0,
-              ua.L, ua.c0, ua.c1, ua.c2, ua.c3, ua.Ts, ua.Tn, ua.V, ua.dpsi,
-              ua.flags);
-  stage_node(k, sa.s, sa.tdt[1], sa.tdt[0], sa.ratio, sa.Vini, sa.slip, sa.psi, sa.ss,
-             sa.FV, sa.Fd, sa.Ys, sa.Yp, u, K, P);
+  // the next stage's inputs that the update leaves unchanged, loaded before
+  // it (after its stores the loads could not be issued early: the pointers
+  // may alias); the rates of the stage just solved (index s - 1 of FV and
+  // Fd, the update's V and dpsi) come from the update itself
+  const int s = sa.s;  // 2 .. 6
+  const double t = sa.tdt[0], dt = sa.tdt[1];
+  const double sl = sa.slip[k], ps = sa.psi[k], ssk = sa.ss[k];
+  double fv[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  double fd[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int jj = 1; jj < 5; jj++) {
+    if (jj < s - 1) {
+      fv[jj] = sa.FV[jj][k];
+      fd[jj] = sa.Fd[jj][k];
+    }
+  }
+  const NodeRates r = update_node(k, u, P, ua.mu_f, ua.tau_n0, ua.Yp, ua.a, ua.b, ua.eta,
+                                  ua.V0, ua.f0, ua.L, ua.c0, ua.c1, ua.c2, ua.c3, ua.Ts,
+                                  ua.Tn, ua.V, ua.dpsi, ua.flags);
+#pragma unroll
+  for (int jj = 1; jj < 6; jj++) {
+    if (jj == s - 1) {
+      fv[jj] = r.V;
+      fd[jj] = r.dpsi;
+    }
+  }
+  stage_node_in(k, s, dt, t, sa.ratio, sa.Vini, sl, ps, ssk, fv, fd, sa.Ys, sa.Yp, u, K,
+                P);
 }
 
 // The fault update after an accepted step's final stage-1 solve, then the
@@ -305,11 +360,16 @@ __global__ void k_update_next(double *u, int nk, int nz, int K, int P, UpdateArg
     sa.Fd[s][k] = 0.0;
   }
   if (k < 1 || k > nk) return;
-  update_node(k, u, P, ua.mu_f, ua.tau_n0, ua.Yp, ua.a, ua.b, ua.eta, ua.V0, ua.f0,
-              ua.L, ua.c0, ua.c1, ua.c2, ua.c3, ua.Ts, ua.Tn, ua.V, ua.dpsi,
-              ua.flags);
-  stage_node(k, 2, dt, t, sa.ratio, sa.Vini, sa.slip, sa.psi, sa.ss, sa.FV, sa.Fd,
-             sa.Ys, sa.Yp, u, K, P);
+  // stage 2 reads slip, psi, SlipShift (loaded before the update, as in
+  // k_update_stage) and F_1, the rates the update forms
+  const double sl = sa.slip[k], ps = sa.psi[k], ssk = sa.ss[k];
+  const NodeRates r = update_node(k, u, P, ua.mu_f, ua.tau_n0, ua.Yp, ua.a, ua.b, ua.eta,
+                                  ua.V0, ua.f0, ua.L, ua.c0, ua.c1, ua.c2, ua.c3, ua.Ts,
+                                  ua.Tn, ua.V, ua.dpsi, ua.flags);
+  double fv[6] = {0.0, r.V, 0.0, 0.0, 0.0, 0.0};
+  double fd[6] = {0.0, r.dpsi, 0.0, 0.0, 0.0, 0.0};
+  stage_node_in(k, 2, dt, t, sa.ratio, sa.Vini, sl, ps, ssk, fv, fd, sa.Ys, sa.Yp, u, K,
+                P);
 }
 
 __global__ void k_zero_rates(double *const *FV, double *const *Fd, int nz,
