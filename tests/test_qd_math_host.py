@@ -7,7 +7,9 @@ code the fault kernels run) built for the host with g++ -ffp-contract=off
   called separately, at every branch edge of the two functions, at the
   infinities and at NaN;
 * every function of the header gives the bits of the oracle's copy
-  (oracle/qdeq_oracle.c), which the GPU-vs-oracle parity tests rely on.
+  (oracle/qdeq_oracle.c), which the GPU-vs-oracle parity tests rely on;
+* a header with one planted mistake (built in a temporary directory) fails
+  each of these comparisons.
 """
 import math
 import os
@@ -33,17 +35,22 @@ def _val(b: int) -> float:
     return struct.unpack("<d", struct.pack("<Q", b))[0]
 
 
-@pytest.fixture(scope="module")
-def check_bin(tmp_path_factory):
+CSRC = os.path.join(ROOT, "qdeqcycle_fdm_amd", "csrc")
+
+
+def _build(out: str, include_dir: str) -> str:
     gxx = shutil.which("g++")
     if gxx is None or not os.path.isdir("/opt/rocm/include"):
         pytest.skip("g++ or the ROCm headers are missing")
-    out = str(tmp_path_factory.mktemp("qdm") / "qd_math_check")
     subprocess.run([gxx, "-O2", "-std=c++17", "-ffp-contract=off", "-D__HIP_PLATFORM_AMD__",
-                    "-I/opt/rocm/include",
-                    "-I", os.path.join(ROOT, "qdeqcycle_fdm_amd", "csrc"), SRC, "-o", out],
+                    "-I/opt/rocm/include", "-I", include_dir, SRC, "-o", out],
                    check=True, capture_output=True)
     return out
+
+
+@pytest.fixture(scope="module")
+def check_bin(tmp_path_factory):
+    return _build(str(tmp_path_factory.mktemp("qdm") / "qd_math_check"), CSRC)
 
 
 def _inputs():
@@ -85,3 +92,26 @@ def test_engine_header_equals_oracle_copies(check_bin):
     for x, row in zip(xs, _run(check_bin, xs)):
         for i, name in enumerate(NAMES):
             assert _same(row[i], _bits(oracle.elem(name, x))), (name, x)
+
+
+@pytest.mark.parametrize("old,new,col,ref", [
+    # sinh of the shared branch given cosh's sum: caught against qd_sinh
+    ("      r = 0.5 * (e - ie);", "      r = 0.5 * (e + ie);", 6, 1),
+    # one Taylor coefficient of exp changed: caught against the oracle's exp
+    ("  p = p * r + 1.0 / 720.0;", "  p = p * r + 1.0 / 721.0;", 0, None),
+])
+def test_planted_mistakes_are_caught(tmp_path, check_bin, old, new, col, ref):
+    """Each comparison above fails for a header with one planted mistake."""
+    text = open(os.path.join(CSRC, "qd_math.h")).read()
+    assert text.count(old) == 1
+    d = tmp_path / "inc"
+    d.mkdir()
+    (d / "qd_math.h").write_text(text.replace(old, new))
+    mutant = _build(str(tmp_path / "mutant"), str(d))
+    xs = _inputs()
+    rows = _run(mutant, xs)
+    if ref is not None:
+        bad = [x for x, r in zip(xs, rows) if not _same(r[col], r[ref])]
+    else:
+        bad = [x for x, r in zip(xs, rows) if not _same(r[col], _bits(oracle.elem("exp", x)))]
+    assert bad, "the planted mistake went unnoticed"
